@@ -1,0 +1,109 @@
+// Shared definitions of the D3Q19 engine: lattice tables, grid description, mask bits,
+// kernel argument blocks.  Included by host and device translation units.
+#pragma once
+
+#include <cstdint>
+
+namespace nlos {
+
+constexpr int Q = 19;
+
+// Velocity ordering B of the reference (05_sensor_illusion.cpp:21-23).
+constexpr int EX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 0, 0, 0, 0};
+constexpr int EY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1};
+constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+// inv_q as the search of 05:301-312 yields it: e_INV[q] = -e_q.
+constexpr int INV[Q] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+
+// Weights (05:25-31).
+constexpr double W64[Q] = {
+    1.0 / 3.0,
+    1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+constexpr double CS2 = 1.0 / 3.0;                  // 05:17
+constexpr double CS = 0.57735026918962576451;      // 05:18
+constexpr double PI = 3.14159265358979323846;      // 05:19
+
+// Storage modes.  F64: full populations f, FP64, reference arithmetic order.  F32/F16:
+// deviations g = f - w_q, FP32 arithmetic, stored in FP32 or in FP16 times 2^k.
+enum Mode : int { M_F64 = 0, M_F32 = 1, M_F16 = 2 };
+
+// Mask byte bits.  The array covers the halo: ground halo cells are SOLID, the other halo
+// cells are fluid (05:509-531).
+constexpr uint8_t MK_SOLID = 1;   // CellType SOLID (05:36)
+constexpr uint8_t MK_NEAR = 2;    // fluid cell with a solid cell among its 18 neighbours
+constexpr uint8_t MK_PROBE = 4;   // fluid or solid cell that carries a probe
+
+// Face bits as in nlos_engine.h.
+constexpr int F_XM = 1, F_XP = 2, F_YM = 4, F_YP = 8, F_ZM = 16, F_ZP = 32;
+
+// Grid: interior nx*ny*nz, one halo layer, x pitch px >= nx+2.  Cell (i,j,k) with i in
+// [0,nx+1] etc. sits at i + px*(j + py*k); population q of it at q*ncell + cell.
+struct GridDesc {
+    int nx, ny, nz;
+    int px, py, pz;
+    long long plane;   // px*py
+    long long ncell;   // px*py*pz
+};
+
+// Lattice layouts of the in-place (A-A) scheme.  Post-collision population q of cell x:
+//   LAY_P (after an even-type sweep, and initially): slot INV[q] of x itself;
+//   LAY_Q (after an odd-type sweep): slot q of x+e_q, or slot INV[q] of x when x+e_q is
+//   solid (the half-way bounce-back is folded into the store).
+// An odd-type sweep turns P into Q, an even-type sweep Q into P.
+enum Layout : int { LAY_P = 0, LAY_Q = 1 };
+
+struct StepArgs {
+    void *pop;
+    const uint8_t *mask;
+    const void *sig_x, *sig_y, *sig_z;   // per-axis sponge tables (compute type)
+    GridDesc g;
+    double omega;
+    double alpha;                        // wall reflection, 1 = reference
+    float scale, inv_scale;              // F16 storage scale and its inverse
+    const long long *probe_cells;        // sorted unique probe cells
+    int n_probe_cells;
+    double *probe_row;                   // this step's row of the probe history
+    float *field;                        // interior rho-1 (capture sweeps only)
+    int *flag;                           // bit 0: non-finite density seen
+    int tiles_x, tiles_y;                // block tiling of the interior
+    int nblocks;
+    int xcd_remap;                       // map consecutive tiles to one XCD
+};
+
+struct GhostArgs {
+    void *pop;
+    const uint8_t *mask;
+    GridDesc g;
+    int nscbc_faces;
+    const void *s_old;                    // tracked state of solid cells on open faces
+    const int *face_map[6];               // per NSCBC face: plane index -> state slot or -1
+    float scale, inv_scale;
+    long long n_halo;
+};
+
+struct SolidArgs {
+    void *pop;
+    const uint8_t *mask;
+    GridDesc g;
+    const long long *cells;               // boundary solid cells
+    int n;
+    const void *s_old;
+    void *s_new;
+    double alpha;
+    float scale, inv_scale;
+};
+
+struct SourceArgs {
+    void *pop;
+    const uint8_t *mask;
+    GridDesc g;
+    const long long *cells;               // fluid cells of this step's stencil
+    const double *adds;                   // A*ww per cell (05:660)
+    int n;
+    float scale, inv_scale;
+};
+
+}  // namespace nlos

@@ -1,0 +1,1110 @@
+// C-ABI implementation of the engine (include/nlos_engine.h): configuration, geometry,
+// source schedule, probe and field output, checkpoints, and the per-step launch sequence.
+// Host arithmetic that must round like the reference (grid sizes, dt, sponge ramp,
+// source weights, STL transform) is written as in 05_sensor_illusion.cpp and built with
+// -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/nlos_engine.h"
+#include "common.hpp"
+#include "launch.hpp"
+
+#define NLOS_API extern "C" __attribute__((visibility("default")))
+
+using namespace nlos;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(NLOS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                 \
+    } while (0)
+
+// Device buffer that grows on demand.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; p = nullptr; bytes = 0; }
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Triangle13 { double v[13]; };   // v1, v2, v3, min x, min y, max x, max y
+
+// Binary STL parse and 200x200 binning (GeometryEngine::load/buildGrid, 05:70-142).
+struct HostGeometry {
+    std::vector<Triangle13> tris;
+    double gmin[3] = {1e30, 1e30, 1e30}, gmax[3] = {-1e30, -1e30, -1e30};
+    int rows = 200, cols = 200;
+    double cell_w = 1.0, cell_h = 1.0;
+    std::vector<int> bin_start, bin_tris;
+
+    bool parse(const unsigned char *buf, size_t len, double scale, double sx, double sy, double sz) {
+        tris.clear();
+        if (len < 84) return false;
+        uint32_t n = 0;
+        std::memcpy(&n, buf + 80, 4);
+        if (len < 84 + (size_t)n * 50) return false;
+        tris.resize(n);
+        for (int a = 0; a < 3; ++a) { gmin[a] = 1e30; gmax[a] = -1e30; }
+        for (uint32_t t = 0; t < n; ++t) {
+            float rec[12];
+            std::memcpy(rec, buf + 84 + (size_t)t * 50, 48);
+            double v[9];
+            const double sh[3] = {sx, sy, sz};
+            for (int p = 0; p < 3; ++p)
+                for (int a = 0; a < 3; ++a) v[3 * p + a] = rec[3 + 3 * p + a] * scale + sh[a];
+            Triangle13 &T = tris[t];
+            for (int a = 0; a < 9; ++a) T.v[a] = v[a];
+            double mn[3], mx[3];
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = std::min({v[a], v[3 + a], v[6 + a]});
+                mx[a] = std::max({v[a], v[3 + a], v[6 + a]});
+                gmin[a] = std::min(gmin[a], mn[a]);
+                gmax[a] = std::max(gmax[a], mx[a]);
+            }
+            T.v[9] = mn[0]; T.v[10] = mn[1]; T.v[11] = mx[0]; T.v[12] = mx[1];
+        }
+        build_bins();
+        return true;
+    }
+
+    void build_bins() {
+        bin_start.assign((size_t)rows * cols + 1, 0);
+        bin_tris.clear();
+        if (tris.empty()) return;
+        cell_w = (gmax[0] - gmin[0]) / rows;
+        cell_h = (gmax[1] - gmin[1]) / cols;
+        if (cell_w < 1e-9) cell_w = 1.0;
+        if (cell_h < 1e-9) cell_h = 1.0;
+        std::vector<int> r1(tris.size()), r2(tris.size()), c1(tris.size()), c2(tris.size());
+        std::vector<int> count((size_t)rows * cols, 0);
+        for (size_t t = 0; t < tris.size(); ++t) {
+            const Triangle13 &T = tris[t];
+            int a = (int)((T.v[9] - gmin[0]) / cell_w), b = (int)((T.v[11] - gmin[0]) / cell_w);
+            int c = (int)((T.v[10] - gmin[1]) / cell_h), d = (int)((T.v[12] - gmin[1]) / cell_h);
+            r1[t] = std::max(0, std::min(rows - 1, a));
+            r2[t] = std::max(0, std::min(rows - 1, b));
+            c1[t] = std::max(0, std::min(cols - 1, c));
+            c2[t] = std::max(0, std::min(cols - 1, d));
+            for (int r = r1[t]; r <= r2[t]; ++r)
+                for (int cc = c1[t]; cc <= c2[t]; ++cc) ++count[(size_t)r * cols + cc];
+        }
+        for (size_t b = 0; b < count.size(); ++b) bin_start[b + 1] = bin_start[b] + count[b];
+        bin_tris.resize(bin_start.back());
+        std::vector<int> fill(bin_start.begin(), bin_start.end() - 1);
+        for (size_t t = 0; t < tris.size(); ++t)
+            for (int r = r1[t]; r <= r2[t]; ++r)
+                for (int cc = c1[t]; cc <= c2[t]; ++cc) bin_tris[fill[(size_t)r * cols + cc]++] = (int)t;
+    }
+};
+
+struct ProbeDesc { double x, y, z; int gi, gj, gk; int unique; bool in_solid; };
+
+}  // namespace
+
+struct nlos_engine {
+    nlos_config cfg{};
+    int mode = M_F64;
+    GridDesc g{};
+    double dt = 0.0, omega = 0.0;
+    float scale = 1.0f, inv_scale = 1.0f;
+    size_t s_bytes = 8, c_bytes = 8;   // storage / compute element sizes
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    DevBuf pop, mask, sig[3], flag, field, hist, probe_cells, s_a, s_b, solid_cells, src_cells,
+        src_adds, face_map[6], gh, vox_tri, vox_bins, vox_start, rho_tmp;
+    bool field_valid = false;
+    bool field_alloc = false;
+
+    std::vector<uint8_t> hmask;     // interior SOLID flags, x fastest
+    std::vector<ProbeDesc> probes;
+    std::vector<long long> probe_ucells;
+    int n_solid = 0;
+    long long hist_cap = 0;         // steps
+    long long step = 0;
+    int lay = LAY_P;
+
+    nlos_source src{};
+    std::vector<double> samples;
+    bool has_source = false;
+
+    dim3 block{64, 4, 1};
+    int xcd_remap = 1;
+
+    bool timing = false;
+    std::vector<hipEvent_t> ev;     // start/stop pairs
+    size_t ev_used = 0;
+    long long timed_launches = 0;
+    double timed_ms = 0.0;
+
+    std::vector<long long> sched_off;
+    std::vector<long long> sched_cells;
+    std::vector<double> sched_adds;
+
+    long long n_interior() const { return (long long)g.nx * g.ny * g.nz; }
+    long long cell(int i, int j, int k) const {
+        return (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    }
+    bool solid_interior(int gi, int gj, int gk) const {
+        return hmask[(size_t)gi + (size_t)g.nx * ((size_t)gj + (size_t)g.ny * gk)] != 0;
+    }
+};
+
+namespace {
+
+// ---- dispatch over storage modes ------------------------------------------------------
+template <class F>
+hipError_t by_mode(int mode, F &&f) {
+    switch (mode) {
+    case M_F64: return f(Launch<M_F64>{});
+    case M_F32: return f(Launch<M_F32>{});
+    default: return f(Launch<M_F16>{});
+    }
+}
+
+// Per-axis sponge ramp, evaluated as in 05:491-506: sigma(min d) = max over axes of
+// sigma(d_axis) because the ramp is monotone in d and rounding is monotone.
+std::vector<double> sponge_axis(const nlos_config &c, int n, double lo, double hi, int bit_lo, int bit_hi) {
+    std::vector<double> s((size_t)n + 2, 0.0);
+    for (int i = 0; i < n + 2; ++i) {
+        const double x = lo + ((i - 1) + 0.5) * c.dx;
+        double d = 1e30;
+        if (c.sponge_faces & bit_lo) d = std::min(d, x - lo);
+        if (c.sponge_faces & bit_hi) d = std::min(d, hi - x);
+        if (d < c.sponge_width) {
+            const double r = (c.sponge_width - d) / c.sponge_width;
+            s[i] = c.sponge_strength * r * r;
+        }
+    }
+    return s;
+}
+
+int upload_sponge(nlos_engine *e) {
+    const nlos_config &c = e->cfg;
+    std::vector<double> ax[3] = {
+        sponge_axis(c, e->g.nx, c.x_min, c.x_max, NLOS_FACE_XM, NLOS_FACE_XP),
+        sponge_axis(c, e->g.ny, c.y_min, c.y_max, NLOS_FACE_YM, NLOS_FACE_YP),
+        sponge_axis(c, e->g.nz, c.z_min, c.z_max, NLOS_FACE_ZM, NLOS_FACE_ZP)};
+    for (int a = 0; a < 3; ++a) {
+        const size_t n = ax[a].size();
+        HIP_TRY(e->sig[a].reserve(n * 8));
+        if (e->mode == M_F64) {
+            HIP_TRY(hipMemcpy(e->sig[a].p, ax[a].data(), n * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(n);
+            for (size_t i = 0; i < n; ++i) f[i] = ax[a][i] > 1e-12 ? (float)ax[a][i] : 0.0f;
+            HIP_TRY(hipMemcpy(e->sig[a].p, f.data(), n * 4, hipMemcpyHostToDevice));
+        }
+    }
+    return NLOS_OK;
+}
+
+// Probe bits, NEAR bits, tracked solid cells on open faces; state reset to rest.
+int finalize_mask(nlos_engine *e) {
+    GridDesc &g = e->g;
+    HIP_TRY(launch_mask_flags(e->mask.as<uint8_t>(), g, e->stream));
+    for (long long c : e->probe_ucells) {
+        uint8_t b = 0;
+        HIP_TRY(hipMemcpyAsync(&b, e->mask.as<uint8_t>() + c, 1, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        b |= MK_PROBE;
+        HIP_TRY(hipMemcpyAsync(e->mask.as<uint8_t>() + c, &b, 1, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    // Solid interior cells on the boundary plane of an NSCBC face: their pulled state
+    // enters the ghost through macro_at (05:689-707), so it is tracked.
+    std::unordered_map<long long, int> slot_of;
+    std::vector<long long> cells;
+    std::vector<int> maps[6];
+    const int nf = e->cfg.nscbc_faces;
+    for (int face = 0; face < 6; ++face) {
+        if (!(nf & (1 << face))) continue;
+        const int axis = face / 2;
+        const bool plus = face & 1;
+        const int na = axis == 0 ? g.ny : g.nx;
+        const int nb = axis == 2 ? g.ny : g.nz;
+        maps[face].assign((size_t)na * nb, -1);
+        for (int b = 0; b < nb; ++b)
+            for (int a = 0; a < na; ++a) {
+                int gi, gj, gk;
+                if (axis == 0) { gi = plus ? g.nx - 1 : 0; gj = a; gk = b; }
+                else if (axis == 1) { gi = a; gj = plus ? g.ny - 1 : 0; gk = b; }
+                else { gi = a; gj = b; gk = plus ? g.nz - 1 : 0; }
+                if (!e->solid_interior(gi, gj, gk)) continue;
+                const long long c = e->cell(gi + 1, gj + 1, gk + 1);
+                auto it = slot_of.find(c);
+                int s;
+                if (it == slot_of.end()) { s = (int)cells.size(); slot_of[c] = s; cells.push_back(c); }
+                else s = it->second;
+                maps[face][(size_t)a + (size_t)na * b] = s;
+            }
+    }
+    e->n_solid = (int)cells.size();
+    for (int face = 0; face < 6; ++face) {
+        if (e->n_solid == 0 || maps[face].empty()) continue;
+        HIP_TRY(e->face_map[face].reserve(maps[face].size() * 4));
+        HIP_TRY(hipMemcpy(e->face_map[face].p, maps[face].data(), maps[face].size() * 4, hipMemcpyHostToDevice));
+    }
+    if (e->n_solid > 0) {
+        const size_t sb = (size_t)e->n_solid * Q * e->c_bytes;
+        HIP_TRY(e->s_a.reserve(sb));
+        HIP_TRY(e->s_b.reserve(sb));
+        HIP_TRY(e->solid_cells.reserve(cells.size() * 8));
+        HIP_TRY(hipMemcpy(e->solid_cells.p, cells.data(), cells.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::fill_state_rest(e->s_a.p, e->n_solid, e->stream); }));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return NLOS_OK;
+}
+
+// Interior SOLID flags of the device mask into e->hmask.
+int pull_host_mask(nlos_engine *e) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<uint8_t> full((size_t)e->g.ncell);
+    HIP_TRY(hipMemcpy(full.data(), e->mask.p, full.size(), hipMemcpyDeviceToHost));
+    e->hmask.assign((size_t)e->n_interior(), 0);
+    size_t n = 0;
+    for (int k = 1; k <= e->g.nz; ++k)
+        for (int j = 1; j <= e->g.ny; ++j) {
+            const uint8_t *row = full.data() + e->cell(1, j, k);
+            for (int i = 0; i < e->g.nx; ++i) e->hmask[n++] = row[i] & MK_SOLID;
+        }
+    return NLOS_OK;
+}
+
+int check_flag(nlos_engine *e) {
+    int f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, e->flag.p, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (f & 1) return fail(NLOS_ERR_NONFINITE, "non-finite density produced during stepping");
+    return NLOS_OK;
+}
+
+int ensure_hist(nlos_engine *e, long long steps) {
+    const long long nu = (long long)e->probe_ucells.size();
+    if (nu == 0 || steps <= e->hist_cap) return NLOS_OK;
+    long long cap = std::max<long long>(steps, std::max<long long>(1024, 2 * e->hist_cap));
+    DevBuf nb;
+    HIP_TRY(nb.reserve((size_t)cap * nu * 8));
+    HIP_TRY(hipMemsetAsync(nb.p, 0, (size_t)cap * nu * 8, e->stream));
+    if (e->hist.p && e->hist_cap > 0)
+        HIP_TRY(hipMemcpyAsync(nb.p, e->hist.p, (size_t)e->hist_cap * nu * 8, hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->hist.release();
+    e->hist = nb;
+    e->hist_cap = cap;
+    return NLOS_OK;
+}
+
+// ---- source schedule (05:576-667, 08:543) ---------------------------------------------
+double waveform(const nlos_engine *e, double t) {
+    const nlos_source &s = e->src;
+    switch (s.waveform) {
+    case NLOS_WAVE_GAUSS_PULSE: return s.amp * std::exp(-0.5 * std::pow((t - s.t0) / s.sigma_t, 2));
+    case NLOS_WAVE_SINE: return s.amp * std::sin(2.0 * PI * s.freq * t);
+    case NLOS_WAVE_SAMPLED: {
+        const long long k = std::llround(t / e->dt);
+        if (k < 0 || k >= (long long)e->samples.size()) return 0.0;
+        return s.amp * e->samples[(size_t)k];
+    }
+    case NLOS_WAVE_RICKER: {
+        double a = PI * s.freq * (t - s.t0);
+        a = a * a;
+        return s.amp * (1.0 - 2.0 * a) * std::exp(-a);
+    }
+    }
+    return 0.0;
+}
+
+// DroneSource::update (05:221-238)
+void source_at(const nlos_source &s, double t, double &cx, double &cy, double &cz) {
+    const double dx = s.ex - s.sx, dy = s.ey - s.sy, dz = s.ez - s.sz;
+    const double dist_total = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist_total < 1e-12 || s.speed < 1e-12) { cx = s.sx; cy = s.sy; cz = s.sz; return; }
+    const double travel_time = dist_total / s.speed;
+    double r = t / travel_time;
+    if (r > 1.0) r = 1.0;
+    cx = s.sx + r * dx;
+    cy = s.sy + r * dy;
+    cz = s.sz + r * dz;
+}
+
+// Appends the stencil cells and adds of time t.
+void schedule_step(nlos_engine *e, double t) {
+    if (!e->has_source) return;
+    const nlos_config &c = e->cfg;
+    const nlos_source &s = e->src;
+    const double DX = c.dx;
+    const double val = waveform(e, t);
+    double cxs, cys, czs;
+    source_at(s, t, cxs, cys, czs);
+    const int NX = e->g.nx, NY = e->g.ny, NZ = e->g.nz;
+    long long cells[512];
+    double ww[512];
+    int n = 0;
+    double sumw = 0.0;
+    if (s.injection == NLOS_INJECT_TRILINEAR) {
+        const double fx = (cxs - c.x_min) / DX - 0.5;
+        const double fy = (cys - c.y_min) / DX - 0.5;
+        const double fz = (czs - c.z_min) / DX - 0.5;
+        const int i0 = (int)std::floor(fx), j0 = (int)std::floor(fy), k0 = (int)std::floor(fz);
+        const double tx = fx - i0, ty = fy - j0, tz = fz - k0;
+        for (int q = 0; q < 8; ++q) {
+            const int di = q & 1, dj = (q >> 1) & 1, dk = (q >> 2) & 1;
+            const int gi = i0 + di, gj = j0 + dj, gk = k0 + dk;
+            const double wt = (di ? tx : 1.0 - tx) * (dj ? ty : 1.0 - ty) * (dk ? tz : 1.0 - tz);
+            if (gi < 0 || gi >= NX || gj < 0 || gj >= NY || gk < 0 || gk >= NZ) continue;
+            if (e->solid_interior(gi, gj, gk)) continue;
+            sumw += wt;
+            cells[n] = e->cell(gi + 1, gj + 1, gk + 1);
+            ww[n++] = wt;
+        }
+    } else {
+        const int R = std::min(s.stencil_radius, 3);
+        const double sigma2 = (s.stencil_sigma_cells * DX) * (s.stencil_sigma_cells * DX);
+        const int ic = (int)std::floor((cxs - c.x_min) / DX);
+        const int jc = (int)std::floor((cys - c.y_min) / DX);
+        const int kc = (int)std::floor((czs - c.z_min) / DX);
+        if (ic < -R || ic >= NX + R || jc < -R || jc >= NY + R || kc < -R || kc >= NZ + R) return;
+        for (int gk = kc - R; gk <= kc + R; ++gk)
+            for (int gj = jc - R; gj <= jc + R; ++gj)
+                for (int gi = ic - R; gi <= ic + R; ++gi) {
+                    if (gi < 0 || gi >= NX || gj < 0 || gj >= NY || gk < 0 || gk >= NZ) continue;
+                    const double x = c.x_min + (gi + 0.5) * DX;
+                    const double y = c.y_min + (gj + 0.5) * DX;
+                    const double z = c.z_min + (gk + 0.5) * DX;
+                    const double dx = x - cxs, dy = y - cys, dz = z - czs;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double w = std::exp(-0.5 * r2 / sigma2);
+                    if (e->solid_interior(gi, gj, gk)) continue;
+                    sumw += w;
+                    cells[n] = e->cell(gi + 1, gj + 1, gk + 1);
+                    ww[n++] = w;
+                }
+    }
+    if (sumw < 1e-30) return;
+    const double A = (val * e->dt * s.gain) / sumw;
+    for (int q = 0; q < n; ++q) {
+        e->sched_cells.push_back(cells[q]);
+        e->sched_adds.push_back(A * ww[q]);
+    }
+}
+
+int upload_schedule(nlos_engine *e) {
+    const size_t n = e->sched_cells.size();
+    if (n == 0) return NLOS_OK;
+    HIP_TRY(e->src_cells.reserve(n * 8));
+    HIP_TRY(e->src_adds.reserve(n * 8));
+    HIP_TRY(hipMemcpyAsync(e->src_cells.p, e->sched_cells.data(), n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->src_adds.p, e->sched_adds.data(), n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return NLOS_OK;
+}
+
+// ---- launch helpers -------------------------------------------------------------------
+int launch_source(nlos_engine *e, long long off, long long n) {
+    if (n <= 0) return NLOS_OK;
+    SourceArgs a{};
+    a.pop = e->pop.p;
+    a.mask = e->mask.as<uint8_t>();
+    a.g = e->g;
+    a.cells = e->src_cells.as<long long>() + off;
+    a.adds = e->src_adds.as<double>() + off;
+    a.n = (int)n;
+    a.scale = e->scale;
+    a.inv_scale = e->inv_scale;
+    HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::source(a, e->lay, e->stream); }));
+    return NLOS_OK;
+}
+
+int launch_halos(nlos_engine *e) {
+    GhostArgs a{};
+    a.pop = e->pop.p;
+    a.mask = e->mask.as<uint8_t>();
+    a.g = e->g;
+    a.nscbc_faces = e->cfg.nscbc_faces;
+    a.s_old = e->s_a.p;
+    for (int f = 0; f < 6; ++f) a.face_map[f] = e->face_map[f].as<int>();
+    a.scale = e->scale;
+    a.inv_scale = e->inv_scale;
+    const long long nx = e->g.nx, ny = e->g.ny, nz = e->g.nz;
+    a.n_halo = 2 * (nx + 2) * (ny + 2) + 2 * (nx + 2) * nz + 2 * ny * nz;
+    HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::ghost(a, e->lay, e->stream); }));
+    if (e->n_solid > 0) {
+        SolidArgs s{};
+        s.pop = e->pop.p;
+        s.mask = e->mask.as<uint8_t>();
+        s.g = e->g;
+        s.cells = e->solid_cells.as<long long>();
+        s.n = e->n_solid;
+        s.s_old = e->s_a.p;
+        s.s_new = e->s_b.p;
+        s.alpha = e->cfg.wall_reflect;
+        s.scale = e->scale;
+        s.inv_scale = e->inv_scale;
+        HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::solid(s, e->stream); }));
+        std::swap(e->s_a, e->s_b);
+    }
+    return NLOS_OK;
+}
+
+int launch_sweep(nlos_engine *e, bool capture) {
+    if (capture) {
+        if (!e->field_alloc) {
+            HIP_TRY(e->field.reserve((size_t)e->n_interior() * 4));
+            e->field_alloc = true;
+        }
+        HIP_TRY(launch_fill_f32(e->field.as<float>(), e->n_interior(), 0.0f, e->stream));
+    }
+    StepArgs a{};
+    a.pop = e->pop.p;
+    a.mask = e->mask.as<uint8_t>();
+    a.sig_x = e->sig[0].p;
+    a.sig_y = e->sig[1].p;
+    a.sig_z = e->sig[2].p;
+    a.g = e->g;
+    a.omega = e->omega;
+    a.alpha = e->cfg.wall_reflect;
+    a.scale = e->scale;
+    a.inv_scale = e->inv_scale;
+    a.probe_cells = e->probe_cells.as<long long>();
+    a.n_probe_cells = (int)e->probe_ucells.size();
+    a.probe_row = a.n_probe_cells ? e->hist.as<double>() + (size_t)e->step * a.n_probe_cells : nullptr;
+    a.field = capture ? e->field.as<float>() : nullptr;
+    a.flag = e->flag.as<int>();
+    a.tiles_x = (e->g.nx + (int)e->block.x - 1) / (int)e->block.x;
+    a.tiles_y = (e->g.ny + (int)e->block.y - 1) / (int)e->block.y;
+    a.nblocks = a.tiles_x * a.tiles_y * e->g.nz;
+    a.xcd_remap = e->xcd_remap;
+    const bool odd = e->lay == LAY_P;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e->timing) {
+        if (e->ev_used + 2 > e->ev.size()) {
+            for (int q = 0; q < 64; ++q) {
+                hipEvent_t x;
+                HIP_TRY(hipEventCreate(&x));
+                e->ev.push_back(x);
+            }
+        }
+        t0 = e->ev[e->ev_used];
+        t1 = e->ev[e->ev_used + 1];
+        e->ev_used += 2;
+        HIP_TRY(hipEventRecord(t0, e->stream));
+    }
+    HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::step(a, odd, capture, e->block, e->stream); }));
+    if (e->timing) HIP_TRY(hipEventRecord(t1, e->stream));
+    e->lay = odd ? LAY_Q : LAY_P;
+    if (capture) e->field_valid = true;
+    ++e->step;
+    return NLOS_OK;
+}
+
+int collect_timing(nlos_engine *e) {
+    if (e->ev_used == 0) return NLOS_OK;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (size_t q = 0; q + 1 < e->ev_used; q += 2) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev[q], e->ev[q + 1]));
+        e->timed_ms += ms;
+        ++e->timed_launches;
+    }
+    e->ev_used = 0;
+    return NLOS_OK;
+}
+
+int ready(const nlos_engine *e) {
+    if (!e) return fail(NLOS_ERR_ARG, "null engine");
+    return NLOS_OK;
+}
+
+int parse_block(const char *s, dim3 &b) {
+    int x = 0, y = 0;
+    if (!s || std::sscanf(s, "%dx%d", &x, &y) != 2) return 0;
+    if (x <= 0 || y <= 0 || x * y != 256 || (x % 64) != 0) return 0;
+    b = dim3(x, y, 1);
+    return 1;
+}
+
+}  // namespace
+
+// =========================================================================================
+NLOS_API int nlos_abi_version(void) { return NLOS_ABI_VERSION; }
+NLOS_API const char *nlos_last_error(void) { return g_err.c_str(); }
+
+NLOS_API int nlos_config_default(nlos_config *c) {
+    if (!c) return fail(NLOS_ERR_ARG, "null config");
+    std::memset(c, 0, sizeof *c);
+    c->x_min = 0.0; c->x_max = 1650.0;           // 05:250-252
+    c->y_min = 0.0; c->y_max = 1150.0;
+    c->z_min = 0.0; c->z_max = 250.0;
+    c->dx = 2.0;                                 // 05:253
+    c->c_phys = 340.0;                           // 05:198
+    c->tau = 0.55;                               // 05:268
+    c->sponge_width = 20.0 * c->dx;              // 05:455
+    c->sponge_strength = 0.35;                   // 05:456
+    c->sky_z = 120.0;                            // 05:458
+    c->wall_reflect = 1.0;
+    c->sponge_faces = NLOS_FACES_REFERENCE;
+    c->nscbc_faces = NLOS_FACES_REFERENCE;       // use_nscbc = true (05:669)
+    c->ground = 1;
+    c->use_geometry = 1;                         // 05:248
+    c->output_world_coords = 1;                  // 05:255
+    c->precision = NLOS_F64;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_source_default(nlos_source *s) {
+    if (!s) return fail(NLOS_ERR_ARG, "null source");
+    std::memset(s, 0, sizeof *s);
+    s->t0 = 0.05;                 // 05:579
+    s->sigma_t = 0.005;           // 05:580
+    s->gain = 1.0;                // 05:583
+    s->stencil_radius = 2;        // 05:587
+    s->stencil_sigma_cells = 1.25;// 05:588
+    s->waveform = NLOS_WAVE_GAUSS_PULSE;
+    s->injection = NLOS_INJECT_GAUSS5;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
+    if (!cfg || !out) return fail(NLOS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!(cfg->dx > 0.0)) return fail(NLOS_ERR_ARG, "dx must be positive");
+    if (!(cfg->tau > 0.5)) return fail(NLOS_ERR_ARG, "tau must exceed 0.5 (got %g)", cfg->tau);
+    if (!(cfg->c_phys > 0.0)) return fail(NLOS_ERR_ARG, "c_phys must be positive");
+    if (cfg->precision < NLOS_F64 || cfg->precision > NLOS_F16S) return fail(NLOS_ERR_ARG, "bad precision");
+    if (!(cfg->wall_reflect >= 0.0 && cfg->wall_reflect <= 1.0)) return fail(NLOS_ERR_ARG, "wall_reflect outside [0,1]");
+    const int NX = (int)((cfg->x_max - cfg->x_min) / cfg->dx + 0.5);   // 05:332-334
+    const int NY = (int)((cfg->y_max - cfg->y_min) / cfg->dx + 0.5);
+    const int NZ = (int)((cfg->z_max - cfg->z_min) / cfg->dx + 0.5);
+    if (NX < 1 || NY < 1 || NZ < 1) return fail(NLOS_ERR_ARG, "empty grid %d x %d x %d", NX, NY, NZ);
+    if ((long long)(NY + 2) * (NZ) > 0x7fffffffLL / 64) return fail(NLOS_ERR_ARG, "grid too large");
+
+    nlos_engine *e = new nlos_engine();
+    e->cfg = *cfg;
+    e->mode = cfg->precision;
+    e->g.nx = NX; e->g.ny = NY; e->g.nz = NZ;
+    e->g.px = NX + 2; e->g.py = NY + 2; e->g.pz = NZ + 2;
+    e->g.plane = (long long)e->g.px * e->g.py;
+    e->g.ncell = e->g.plane * e->g.pz;
+    e->dt = (CS * cfg->dx) / cfg->c_phys;     // 05:200-203
+    e->omega = 1.0 / cfg->tau;                // 05:908
+    e->s_bytes = e->mode == M_F64 ? 8 : (e->mode == M_F32 ? 4 : 2);
+    e->c_bytes = e->mode == M_F64 ? 8 : 4;
+    if (e->mode == M_F16) {
+        const int k = cfg->f16_scale_log2 ? cfg->f16_scale_log2 : 24;
+        e->scale = std::ldexp(1.0f, k);
+        e->inv_scale = std::ldexp(1.0f, -k);
+    }
+    if (const char *b = std::getenv("NLOS_BLOCK")) parse_block(b, e->block);
+    if (const char *x = std::getenv("NLOS_XCD_REMAP")) e->xcd_remap = std::atoi(x) != 0;
+
+    auto bail = [&](int rc) { nlos_destroy(e); return rc; };
+    hipError_t he = hipGetDevice(&e->device);
+    if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(he)));
+    he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)));
+    const size_t popb = (size_t)e->g.ncell * Q * e->s_bytes;
+    if (e->pop.reserve(popb) != hipSuccess)
+        return bail(fail(NLOS_ERR_NOMEM, "cannot allocate %.2f GB of populations", popb / 1e9));
+    if (e->mask.reserve((size_t)e->g.ncell) != hipSuccess || e->flag.reserve(sizeof(int)) != hipSuccess)
+        return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
+    int rc;
+    if ((rc = upload_sponge(e)) != NLOS_OK) return bail(rc);
+    if (hipMemsetAsync(e->flag.p, 0, sizeof(int), e->stream) != hipSuccess)
+        return bail(fail(NLOS_ERR_HIP, "memset flag"));
+    if (by_mode(e->mode, [&](auto L) { return decltype(L)::fill_rest(e->pop.p, e->g, e->stream); }) != hipSuccess)
+        return bail(fail(NLOS_ERR_HIP, "fill_rest launch failed"));
+    // Empty geometry: mask from heights -1e30 everywhere (fluid, ground halo solid).
+    VoxelArgs v{};
+    v.x_min = cfg->x_min; v.y_min = cfg->y_min; v.z_min = cfg->z_min; v.dx = cfg->dx;
+    v.sky_z = cfg->sky_z; v.use_geometry = cfg->use_geometry; v.ground = cfg->ground;
+    const size_t ncol = (size_t)(NX + 2) * (NY + 2);
+    if (e->gh.reserve(ncol * 8) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "heights"));
+    if (launch_column_heights(v, e->g, e->gh.as<double>(), e->stream) != hipSuccess ||
+        launch_mask_from_heights(v, e->g, e->gh.as<double>(), e->mask.as<uint8_t>(), e->stream) != hipSuccess)
+        return bail(fail(NLOS_ERR_HIP, "mask init launch failed"));
+    if ((rc = pull_host_mask(e)) != NLOS_OK) return bail(rc);
+    if ((rc = finalize_mask(e)) != NLOS_OK) return bail(rc);
+    *out = e;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_destroy(nlos_engine *e) {
+    if (!e) return NLOS_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    DevBuf *bufs[] = {&e->pop, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
+                      &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
+                      &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
+                      &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
+                      &e->vox_bins, &e->vox_start, &e->rho_tmp};
+    for (DevBuf *b : bufs) b->release();
+    for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_get_info(const nlos_engine *e, nlos_info *info) {
+    if (int rc = ready(e)) return rc;
+    if (!info) return fail(NLOS_ERR_ARG, "null info");
+    std::memset(info, 0, sizeof *info);
+    info->nx = e->g.nx; info->ny = e->g.ny; info->nz = e->g.nz;
+    info->precision = e->mode;
+    info->dt = e->dt;
+    info->step = e->step;
+    info->n_probes = (int64_t)e->probes.size();
+    info->n_solid_boundary = e->n_solid;
+    info->bytes_per_update = 2.0 * Q * (double)e->s_bytes + 1.0;
+    size_t tot = 0;
+    const DevBuf *bufs[] = {&e->pop, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
+                            &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
+                            &e->src_adds, &e->gh};
+    for (const DevBuf *b : bufs) tot += b->bytes;
+    info->device_bytes = tot;
+    return NLOS_OK;
+}
+
+// ---- geometry --------------------------------------------------------------------------
+NLOS_API int nlos_voxelize_stl(nlos_engine *e, const void *stl, size_t len, double scale, double sx,
+                               double sy, double sz) {
+    if (int rc = ready(e)) return rc;
+    if (e->step != 0) return fail(NLOS_ERR_STATE, "geometry can only change before the first step");
+    HostGeometry geo;
+    const bool ok = stl && geo.parse((const unsigned char *)stl, len, scale, sx, sy, sz);
+    if (!ok) geo = HostGeometry();
+    VoxelArgs v{};
+    v.n_tri = (int)geo.tris.size();
+    v.grid_rows = geo.rows; v.grid_cols = geo.cols;
+    for (int a = 0; a < 3; ++a) { v.geo_min[a] = geo.gmin[a]; v.geo_max[a] = geo.gmax[a]; }
+    v.cell_w = geo.cell_w; v.cell_h = geo.cell_h;
+    v.x_min = e->cfg.x_min; v.y_min = e->cfg.y_min; v.z_min = e->cfg.z_min; v.dx = e->cfg.dx;
+    v.sky_z = e->cfg.sky_z; v.use_geometry = e->cfg.use_geometry; v.ground = e->cfg.ground;
+    if (v.n_tri > 0) {
+        HIP_TRY(e->vox_tri.reserve(geo.tris.size() * sizeof(Triangle13)));
+        HIP_TRY(e->vox_start.reserve(geo.bin_start.size() * 4));
+        HIP_TRY(e->vox_bins.reserve(std::max<size_t>(geo.bin_tris.size(), 1) * 4));
+        HIP_TRY(hipMemcpy(e->vox_tri.p, geo.tris.data(), geo.tris.size() * sizeof(Triangle13), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->vox_start.p, geo.bin_start.data(), geo.bin_start.size() * 4, hipMemcpyHostToDevice));
+        if (!geo.bin_tris.empty())
+            HIP_TRY(hipMemcpy(e->vox_bins.p, geo.bin_tris.data(), geo.bin_tris.size() * 4, hipMemcpyHostToDevice));
+        v.tri = e->vox_tri.as<double>();
+        v.bin_start = e->vox_start.as<int>();
+        v.bin_tris = e->vox_bins.as<int>();
+    }
+    HIP_TRY(launch_column_heights(v, e->g, e->gh.as<double>(), e->stream));
+    HIP_TRY(launch_mask_from_heights(v, e->g, e->gh.as<double>(), e->mask.as<uint8_t>(), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    int rc = pull_host_mask(e);
+    if (rc) return rc;
+    rc = finalize_mask(e);
+    if (rc) return rc;
+    if (!ok) return fail(NLOS_ERR_IO, "STL not loaded, using empty geometry");
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_load_stl(nlos_engine *e, const char *path, double scale, double sx, double sy, double sz) {
+    if (int rc = ready(e)) return rc;
+    std::vector<char> buf;
+    if (path) {
+        std::ifstream f(path, std::ios::binary);
+        if (f) buf.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    }
+    if (buf.empty()) {
+        int rc = nlos_voxelize_stl(e, nullptr, 0, scale, sx, sy, sz);
+        if (rc != NLOS_OK && rc != NLOS_ERR_IO) return rc;
+        return fail(NLOS_ERR_IO, "STL not loaded (%s), using empty geometry", path ? path : "null");
+    }
+    return nlos_voxelize_stl(e, buf.data(), buf.size(), scale, sx, sy, sz);
+}
+
+NLOS_API int nlos_get_ground_heights(nlos_engine *e, double *out) {
+    if (int rc = ready(e)) return rc;
+    if (!out) return fail(NLOS_ERR_ARG, "null output");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(out, e->gh.p, (size_t)(e->g.nx + 2) * (e->g.ny + 2) * 8, hipMemcpyDeviceToHost));
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_set_mask(nlos_engine *e, const uint8_t *m) {
+    if (int rc = ready(e)) return rc;
+    if (!m) return fail(NLOS_ERR_ARG, "null mask");
+    if (e->step != 0) return fail(NLOS_ERR_STATE, "geometry can only change before the first step");
+    std::vector<uint8_t> full((size_t)e->g.ncell, 0);
+    const GridDesc &g = e->g;
+    for (int k = 0; k <= g.nz + 1; ++k)
+        for (int j = 0; j <= g.ny + 1; ++j)
+            for (int i = 0; i <= g.nx + 1; ++i) {
+                uint8_t v;
+                if (k == 0) v = e->cfg.ground ? MK_SOLID : 0;
+                else if (k == g.nz + 1 || i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1) v = 0;
+                else v = m[(size_t)(i - 1) + (size_t)g.nx * ((size_t)(j - 1) + (size_t)g.ny * (k - 1))] ? MK_SOLID : 0;
+                full[(size_t)e->cell(i, j, k)] = v;
+            }
+    HIP_TRY(hipMemcpy(e->mask.p, full.data(), full.size(), hipMemcpyHostToDevice));
+    e->hmask.assign(m, m + e->n_interior());
+    for (auto &b : e->hmask) b = b ? 1 : 0;
+    return finalize_mask(e);
+}
+
+NLOS_API int nlos_get_mask(nlos_engine *e, uint8_t *m) {
+    if (int rc = ready(e)) return rc;
+    if (!m) return fail(NLOS_ERR_ARG, "null mask");
+    std::memcpy(m, e->hmask.data(), e->hmask.size());
+    return NLOS_OK;
+}
+
+// ---- initial state, source, probes ----------------------------------------------------------
+NLOS_API int nlos_set_density(nlos_engine *e, const double *rho) {
+    if (int rc = ready(e)) return rc;
+    if (!rho) return fail(NLOS_ERR_ARG, "null density");
+    if (e->step != 0) return fail(NLOS_ERR_STATE, "initial density can only be set before the first step");
+    const size_t n = (size_t)e->n_interior();
+    HIP_TRY(e->rho_tmp.reserve(n * 8));
+    HIP_TRY(hipMemcpy(e->rho_tmp.p, rho, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(by_mode(e->mode, [&](auto L) {
+        return decltype(L)::set_density(e->pop.p, e->mask.as<uint8_t>(), e->g, e->rho_tmp.as<double>(), e->scale, e->stream);
+    }));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->rho_tmp.release();
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_set_source(nlos_engine *e, const nlos_source *s) {
+    if (int rc = ready(e)) return rc;
+    if (!s) return fail(NLOS_ERR_ARG, "null source");
+    if (s->waveform < NLOS_WAVE_GAUSS_PULSE || s->waveform > NLOS_WAVE_RICKER) return fail(NLOS_ERR_ARG, "bad waveform");
+    if (s->injection != NLOS_INJECT_GAUSS5 && s->injection != NLOS_INJECT_TRILINEAR) return fail(NLOS_ERR_ARG, "bad injection");
+    if (s->injection == NLOS_INJECT_GAUSS5 && (s->stencil_radius < 0 || s->stencil_radius > 3))
+        return fail(NLOS_ERR_ARG, "stencil radius must be in [0,3]");
+    if (s->waveform == NLOS_WAVE_SAMPLED && (s->n_samples < 0 || (s->n_samples > 0 && !s->samples)))
+        return fail(NLOS_ERR_ARG, "sampled waveform needs samples");
+    e->src = *s;
+    e->samples.assign(s->samples ? s->samples : nullptr, s->samples ? s->samples + s->n_samples : nullptr);
+    e->src.samples = nullptr;
+    e->has_source = true;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_set_probes(nlos_engine *e, const double *xyz, int32_t n, uint8_t *in_solid) {
+    if (int rc = ready(e)) return rc;
+    if (n < 0 || (n > 0 && !xyz)) return fail(NLOS_ERR_ARG, "bad probe list");
+    if (e->step != 0) return fail(NLOS_ERR_STATE, "probes can only change before the first step");
+    std::vector<ProbeDesc> ps;
+    for (int p = 0; p < n; ++p) {
+        ProbeDesc d{};
+        d.x = xyz[3 * p]; d.y = xyz[3 * p + 1]; d.z = xyz[3 * p + 2];
+        d.gi = (int)std::floor((d.x - e->cfg.x_min) / e->cfg.dx);     // 05:417-419
+        d.gj = (int)std::floor((d.y - e->cfg.y_min) / e->cfg.dx);
+        d.gk = (int)std::floor((d.z - e->cfg.z_min) / e->cfg.dx);
+        if (d.gi < 0 || d.gi >= e->g.nx || d.gj < 0 || d.gj >= e->g.ny || d.gk < 0 || d.gk >= e->g.nz)
+            return fail(NLOS_ERR_ARG, "probe %d at (%g,%g,%g) lies outside the grid", p, d.x, d.y, d.z);
+        d.in_solid = e->solid_interior(d.gi, d.gj, d.gk);
+        ps.push_back(d);
+    }
+    // Clear old probe bits.
+    for (long long c : e->probe_ucells) {
+        uint8_t b = 0;
+        HIP_TRY(hipMemcpy(&b, e->mask.as<uint8_t>() + c, 1, hipMemcpyDeviceToHost));
+        b &= (uint8_t)~MK_PROBE;
+        HIP_TRY(hipMemcpy(e->mask.as<uint8_t>() + c, &b, 1, hipMemcpyHostToDevice));
+    }
+    std::vector<long long> uc;
+    for (auto &d : ps) uc.push_back(e->cell(d.gi + 1, d.gj + 1, d.gk + 1));
+    std::sort(uc.begin(), uc.end());
+    uc.erase(std::unique(uc.begin(), uc.end()), uc.end());
+    for (auto &d : ps)
+        d.unique = (int)(std::lower_bound(uc.begin(), uc.end(), e->cell(d.gi + 1, d.gj + 1, d.gk + 1)) - uc.begin());
+    e->probes = ps;
+    e->probe_ucells = uc;
+    e->hist.release();
+    e->hist_cap = 0;
+    if (!uc.empty()) {
+        HIP_TRY(e->probe_cells.reserve(uc.size() * 8));
+        HIP_TRY(hipMemcpy(e->probe_cells.p, uc.data(), uc.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (in_solid)
+        for (int p = 0; p < n; ++p) in_solid[p] = ps[p].in_solid ? 1 : 0;
+    return finalize_mask(e);
+}
+
+// ---- stepping -----------------------------------------------------------------------------
+NLOS_API int nlos_apply_source(nlos_engine *e, double t) {
+    if (int rc = ready(e)) return rc;
+    e->sched_cells.clear();
+    e->sched_adds.clear();
+    schedule_step(e, t);
+    int rc = upload_schedule(e);
+    if (rc) return rc;
+    return launch_source(e, 0, (long long)e->sched_cells.size());
+}
+
+NLOS_API int nlos_exchange_halos(nlos_engine *e) {
+    if (int rc = ready(e)) return rc;
+    return launch_halos(e);
+}
+
+NLOS_API int nlos_step_collision(nlos_engine *e) {
+    if (int rc = ready(e)) return rc;
+    int rc = ensure_hist(e, e->step + 1);
+    if (rc) return rc;
+    return launch_sweep(e, true);
+}
+
+NLOS_API int nlos_step(nlos_engine *e, int64_t n, int32_t flags) {
+    if (int rc = ready(e)) return rc;
+    if (n < 0) return fail(NLOS_ERR_ARG, "negative step count");
+    if (n == 0) return NLOS_OK;
+    int rc = ensure_hist(e, e->step + n);
+    if (rc) return rc;
+    const long long chunk = 4096;
+    for (long long base = 0; base < n; base += chunk) {
+        const long long m = std::min(chunk, n - base);
+        e->sched_cells.clear();
+        e->sched_adds.clear();
+        e->sched_off.assign(1, 0);
+        for (long long s = 0; s < m; ++s) {
+            schedule_step(e, (double)(e->step + s) * e->dt);      // t = s*dt (05:1079)
+            e->sched_off.push_back((long long)e->sched_cells.size());
+        }
+        if ((rc = upload_schedule(e))) return rc;
+        for (long long s = 0; s < m; ++s) {
+            if ((rc = launch_source(e, e->sched_off[s], e->sched_off[s + 1] - e->sched_off[s]))) return rc;
+            if ((rc = launch_halos(e))) return rc;
+            const bool cap = (flags & NLOS_STEP_CAPTURE_FIELD) && base + s == n - 1;
+            if ((rc = launch_sweep(e, cap))) return rc;
+        }
+        if (e->timing && (rc = collect_timing(e))) return rc;
+    }
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_synchronize(nlos_engine *e) {
+    if (int rc = ready(e)) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return check_flag(e);
+}
+
+// ---- results ------------------------------------------------------------------------------
+NLOS_API int nlos_read_probes(nlos_engine *e, double *out, int64_t max_steps, int64_t *n_steps) {
+    if (int rc = ready(e)) return rc;
+    const long long ns = std::min<long long>(e->step, max_steps);
+    if (n_steps) *n_steps = ns;
+    if (int rc = check_flag(e)) return rc;
+    const size_t nu = e->probe_ucells.size();
+    if (!out || nu == 0 || ns <= 0) return NLOS_OK;
+    std::vector<double> h((size_t)ns * nu);
+    HIP_TRY(hipMemcpy(h.data(), e->hist.p, h.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < e->probes.size(); ++p)
+        for (long long s = 0; s < ns; ++s) out[(size_t)p * max_steps + s] = h[(size_t)s * nu + e->probes[p].unique];
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_read_field(nlos_engine *e, float *out) {
+    if (int rc = ready(e)) return rc;
+    if (!out) return fail(NLOS_ERR_ARG, "null output");
+    if (!e->field_valid) return fail(NLOS_ERR_STATE, "no field captured (step with NLOS_STEP_CAPTURE_FIELD)");
+    if (int rc = check_flag(e)) return rc;
+    HIP_TRY(hipMemcpy(out, e->field.p, (size_t)e->n_interior() * 4, hipMemcpyDeviceToHost));
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_write_probe_files(nlos_engine *e, const char *dir, const char *const *names) {
+    if (int rc = ready(e)) return rc;
+    if (!names) return fail(NLOS_ERR_ARG, "null names");
+    const long long ns = e->step;
+    std::vector<double> buf((size_t)e->probes.size() * std::max<long long>(ns, 1));
+    int64_t got = 0;
+    if (int rc = nlos_read_probes(e, buf.data(), ns, &got)) return rc;
+    for (size_t p = 0; p < e->probes.size(); ++p) {
+        std::string path = std::string(dir && *dir ? dir : ".") + "/probe_" + names[p] + ".txt";
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f) return fail(NLOS_ERR_IO, "cannot write %s", path.c_str());
+        for (long long s = 0; s < got; ++s) std::fprintf(f, "%.8e\n", buf[(size_t)p * ns + s]);   // 05:1113
+        std::fclose(f);
+    }
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_write_vtk(nlos_engine *e, const char *path) {
+    if (int rc = ready(e)) return rc;
+    if (!path) return fail(NLOS_ERR_ARG, "null path");
+    const long long n = e->n_interior();
+    std::vector<float> v((size_t)n);
+    if (int rc = nlos_read_field(e, v.data())) return rc;
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(NLOS_ERR_IO, "cannot write %s", path);
+    double ox = e->cfg.x_min, oy = e->cfg.y_min, oz = e->cfg.z_min;
+    if (e->cfg.output_world_coords) { ox += e->cfg.world_offset[0]; oy += e->cfg.world_offset[1]; oz += e->cfg.world_offset[2]; }
+    // 05:1045-1053: default stream formatting is %g; values in scientific, precision 6.
+    std::fprintf(f, "# vtk DataFile Version 3.0\nLBM Urban Acoustics V21\nASCII\n");
+    std::fprintf(f, "DATASET STRUCTURED_POINTS\n");
+    std::fprintf(f, "DIMENSIONS %d %d %d\n", e->g.nx, e->g.ny, e->g.nz);
+    std::fprintf(f, "ORIGIN %g %g %g\n", ox, oy, oz);
+    std::fprintf(f, "SPACING %g %g %g\n", e->cfg.dx, e->cfg.dx, e->cfg.dx);
+    std::fprintf(f, "POINT_DATA %zu\n", (size_t)n);
+    std::fprintf(f, "SCALARS PressureAmp float 1\nLOOKUP_TABLE default\n");
+    std::vector<char> line(64);
+    for (long long q = 0; q < n; ++q) std::fprintf(f, "%.6e\n", (double)v[(size_t)q]);
+    std::fclose(f);
+    return NLOS_OK;
+}
+
+// ---- checkpoint / restore -------------------------------------------------------------------
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int32_t version, mode;
+    int32_t nx, ny, nz, lay;
+    int64_t step, ncell, n_solid, n_ucells;
+    uint64_t pop_bytes, state_bytes, hist_bytes;
+};
+}
+
+NLOS_API int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *needed) {
+    if (int rc = ready(e)) return rc;
+    CkptHeader h{};
+    std::memcpy(h.magic, "NLOSCKP1", 8);
+    h.version = 1; h.mode = e->mode;
+    h.nx = e->g.nx; h.ny = e->g.ny; h.nz = e->g.nz; h.lay = e->lay;
+    h.step = e->step; h.ncell = e->g.ncell; h.n_solid = e->n_solid;
+    h.n_ucells = (int64_t)e->probe_ucells.size();
+    h.pop_bytes = (uint64_t)e->g.ncell * Q * e->s_bytes;
+    h.state_bytes = (uint64_t)e->n_solid * Q * e->c_bytes;
+    h.hist_bytes = (uint64_t)e->step * h.n_ucells * 8;
+    const size_t total = sizeof h + h.pop_bytes + h.state_bytes + h.hist_bytes;
+    if (needed) *needed = total;
+    if (!buf) return NLOS_OK;
+    if (cap < total) return fail(NLOS_ERR_ARG, "checkpoint buffer too small (%zu < %zu)", cap, total);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    char *p = static_cast<char *>(buf);
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    HIP_TRY(hipMemcpy(p, e->pop.p, h.pop_bytes, hipMemcpyDeviceToHost));
+    p += h.pop_bytes;
+    if (h.state_bytes) HIP_TRY(hipMemcpy(p, e->s_a.p, h.state_bytes, hipMemcpyDeviceToHost));
+    p += h.state_bytes;
+    if (h.hist_bytes) HIP_TRY(hipMemcpy(p, e->hist.p, h.hist_bytes, hipMemcpyDeviceToHost));
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_restore(nlos_engine *e, const void *buf, size_t len) {
+    if (int rc = ready(e)) return rc;
+    if (!buf || len < sizeof(CkptHeader)) return fail(NLOS_ERR_ARG, "checkpoint too short");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, "NLOSCKP1", 8) != 0) return fail(NLOS_ERR_ARG, "not a checkpoint");
+    if (h.mode != e->mode || h.nx != e->g.nx || h.ny != e->g.ny || h.nz != e->g.nz || h.ncell != e->g.ncell ||
+        h.n_solid != e->n_solid || h.n_ucells != (int64_t)e->probe_ucells.size())
+        return fail(NLOS_ERR_ARG, "checkpoint does not match this engine's grid, geometry or probes");
+    if (len < sizeof h + h.pop_bytes + h.state_bytes + h.hist_bytes) return fail(NLOS_ERR_ARG, "checkpoint truncated");
+    const char *p = static_cast<const char *>(buf) + sizeof h;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(e->pop.p, p, h.pop_bytes, hipMemcpyHostToDevice));
+    p += h.pop_bytes;
+    if (h.state_bytes) HIP_TRY(hipMemcpy(e->s_a.p, p, h.state_bytes, hipMemcpyHostToDevice));
+    p += h.state_bytes;
+    e->step = 0;
+    if (int rc = ensure_hist(e, h.step)) return rc;
+    if (h.hist_bytes) HIP_TRY(hipMemcpy(e->hist.p, p, h.hist_bytes, hipMemcpyHostToDevice));
+    e->step = h.step;
+    e->lay = h.lay;
+    e->field_valid = false;
+    HIP_TRY(hipMemsetAsync(e->flag.p, 0, sizeof(int), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return NLOS_OK;
+}
+
+// ---- measurement ------------------------------------------------------------------------------
+NLOS_API int nlos_timing_enable(nlos_engine *e, int32_t on) {
+    if (int rc = ready(e)) return rc;
+    e->timing = on != 0;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_timing_reset(nlos_engine *e) {
+    if (int rc = ready(e)) return rc;
+    if (int rc = collect_timing(e)) return rc;
+    e->timed_launches = 0;
+    e->timed_ms = 0.0;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_timing_get(nlos_engine *e, nlos_timing *t) {
+    if (int rc = ready(e)) return rc;
+    if (!t) return fail(NLOS_ERR_ARG, "null timing");
+    if (int rc = collect_timing(e)) return rc;
+    t->launches = e->timed_launches;
+    t->total_ms = e->timed_ms;
+    t->avg_ms = e->timed_launches ? e->timed_ms / (double)e->timed_launches : 0.0;
+    return NLOS_OK;
+}
+
+// Test hook (nlos_engine_test.h): FP64 full populations f* of every interior cell, AoS
+// [cell][q], NaN on solid cells.
+NLOS_API int nlos_debug_read_f(nlos_engine *e, double *out) {
+    if (int rc = ready(e)) return rc;
+    DevBuf tmp;
+    HIP_TRY(tmp.reserve((size_t)e->n_interior() * Q * 8));
+    HIP_TRY(by_mode(e->mode, [&](auto L) {
+        return decltype(L)::gather_f(e->pop.p, e->mask.as<uint8_t>(), e->g, e->lay, e->inv_scale, tmp.as<double>(), e->stream);
+    }));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(out, tmp.p, (size_t)e->n_interior() * Q * 8, hipMemcpyDeviceToHost));
+    tmp.release();
+    return NLOS_OK;
+}
+
+// Test hook (nlos_engine_test.h): overwrite one stored population of one interior cell (storage units are
+// converted from the FP64 full population).  Used to force a non-finite value from the host.
+NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, int32_t q, double f) {
+    if (int rc = ready(e)) return rc;
+    if (i < 0 || i >= e->g.nx || j < 0 || j >= e->g.ny || k < 0 || k >= e->g.nz || q < 0 || q >= Q)
+        return fail(NLOS_ERR_ARG, "poke outside grid");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const long long idx = (long long)q * e->g.ncell + e->cell(i + 1, j + 1, k + 1);
+    if (e->mode == M_F64) {
+        HIP_TRY(hipMemcpy(e->pop.as<double>() + idx, &f, 8, hipMemcpyHostToDevice));
+    } else if (e->mode == M_F32) {
+        const float v = (float)(f - W64[q]);
+        HIP_TRY(hipMemcpy(e->pop.as<float>() + idx, &v, 4, hipMemcpyHostToDevice));
+    } else {
+        const _Float16 v = (_Float16)((float)(f - W64[q]) * e->scale);
+        HIP_TRY(hipMemcpy(static_cast<_Float16 *>(e->pop.p) + idx, &v, 2, hipMemcpyHostToDevice));
+    }
+    return NLOS_OK;
+}

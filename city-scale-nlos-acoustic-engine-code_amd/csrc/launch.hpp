@@ -1,0 +1,58 @@
+// Host-side launch interface of the per-storage-mode kernel translation units
+// (kernels_f64.hip, kernels_f32.hip, kernels_f16.hip).  The engine sees only these
+// declarations; each mode's TU instantiates Launch<MODE>.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace nlos {
+
+template <int MODE>
+struct Launch {
+    // Fused collide-stream sweep.  odd: odd-type (LAY_P -> LAY_Q) else even-type.
+    static hipError_t step(const StepArgs &a, bool odd, bool capture, dim3 block, hipStream_t s);
+    // Ghost layer of the open faces (NSCBC or rest state) written into the current layout.
+    static hipError_t ghost(const GhostArgs &a, int lay, hipStream_t s);
+    // Pull step of solid cells on open faces (their state feeds the NSCBC ghosts).
+    static hipError_t solid(const SolidArgs &a, hipStream_t s);
+    // Source injection f += w_q * add on the stencil cells of one step.
+    static hipError_t source(const SourceArgs &a, int lay, hipStream_t s);
+    // Every population of every cell (halo and padding included) to the rest state.
+    static hipError_t fill_rest(void *pop, const GridDesc &g, hipStream_t s);
+    // f = feq(rho, 0) on fluid interior cells; rho is interior, x fastest.
+    static hipError_t set_density(void *pop, const uint8_t *mask, const GridDesc &g,
+                                  const double *rho, float scale, hipStream_t s);
+    // Rest state of n tracked solid cells.
+    static hipError_t fill_state_rest(void *state, int n, hipStream_t s);
+    // Gather f* of every interior cell into AoS FP64 full populations [cell][q] over the
+    // interior, x fastest (for checkpoints and tests).
+    static hipError_t gather_f(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
+                               float inv_scale, double *out, hipStream_t s);
+};
+
+// Mode-independent helpers (kernels_common.hip).
+hipError_t launch_mask_flags(uint8_t *mask, const GridDesc &g, hipStream_t s);
+hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t *p, long long n, uint8_t v, hipStream_t s);
+
+// Voxeliser (voxel.hip).
+struct VoxelArgs {
+    const double *tri;         // per triangle: v1, v2, v3 (9 doubles), min xy, max xy (4)
+    const int *bin_start;      // CSR over gridRows*gridCols bins
+    const int *bin_tris;
+    int n_tri;
+    int grid_rows, grid_cols;
+    double geo_min[3], geo_max[3];
+    double cell_w, cell_h;
+    double x_min, y_min, z_min, dx;
+    double sky_z;
+    int use_geometry;
+    int ground;
+};
+hipError_t launch_column_heights(const VoxelArgs &v, const GridDesc &g, double *gh, hipStream_t s);
+hipError_t launch_mask_from_heights(const VoxelArgs &v, const GridDesc &g, const double *gh,
+                                    uint8_t *mask, hipStream_t s);
+
+}  // namespace nlos

@@ -1,0 +1,150 @@
+// Column top-hit voxeliser and the mask rule of init_domain, on the GPU, in FP64 with
+// contraction off (built with -ffp-contract=off) so that every hit height rounds as in
+// GeometryEngine::getGroundHeight (05_sensor_illusion.cpp:144-192).  The maximum over a
+// bin's candidate triangles does not depend on their order, so the heights, and the mask
+// derived from them, are bit-identical to the reference's.
+//
+// Also holds the mode-independent helper kernels (neighbour flags, fills).
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace nlos {
+
+// One thread per column (i,j) over (nx+2)*(ny+2), halo columns included (05:460-467).
+__global__ __launch_bounds__(256) void k_column_heights(VoxelArgs v, GridDesc g, double *gh) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ncol = (long long)(g.nx + 2) * (g.ny + 2);
+    if (t >= ncol) return;
+    const int i = (int)(t % (g.nx + 2));
+    const int j = (int)(t / (g.nx + 2));
+    const double x = v.x_min + ((i - 1) + 0.5) * v.dx;
+    const double y = v.y_min + ((j - 1) + 0.5) * v.dx;
+    double out = -1e30;
+    do {
+        if (v.n_tri == 0) break;
+        if (x < v.geo_min[0] || x > v.geo_max[0] || y < v.geo_min[1] || y > v.geo_max[1]) break;
+        const int r = (int)((x - v.geo_min[0]) / v.cell_w);
+        const int cc = (int)((y - v.geo_min[1]) / v.cell_h);
+        if (r < 0 || r >= v.grid_rows || cc < 0 || cc >= v.grid_cols) break;
+        const int bin = r * v.grid_cols + cc;
+        const int b0 = v.bin_start[bin], b1 = v.bin_start[bin + 1];
+        // ray origin (x, y, 1e4), direction (0, 0, -1)
+        const double rox = x, roy = y, roz = 10000.0;
+        const double rdx = 0.0, rdy = 0.0, rdz = -1.0;
+        double maxZ = -1e30;
+        for (int n = b0; n < b1; ++n) {
+            const double *tr = v.tri + (long long)v.bin_tris[n] * 13;
+            // bbox pre-filter (05:164-165)
+            if (tr[11] < x - 1e-6 || tr[9] > x + 1e-6) continue;
+            if (tr[12] < y - 1e-6 || tr[10] > y + 1e-6) continue;
+            const double EPS = 1e-9;
+            // Moller-Trumbore as written in 05:169-189
+            const double e1x = tr[3] - tr[0], e1y = tr[4] - tr[1], e1z = tr[5] - tr[2];
+            const double e2x = tr[6] - tr[0], e2y = tr[7] - tr[1], e2z = tr[8] - tr[2];
+            const double hx = rdy * e2z - rdz * e2y, hy = rdz * e2x - rdx * e2z, hz = rdx * e2y - rdy * e2x;
+            const double a = e1x * hx + e1y * hy + e1z * hz;
+            if (fabs(a) < EPS) continue;
+            const double f = 1.0 / a;
+            const double sx = rox - tr[0], sy = roy - tr[1], sz = roz - tr[2];
+            const double u = f * (sx * hx + sy * hy + sz * hz);
+            if (u < 0.0 || u > 1.0) continue;
+            const double qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+            const double vv = f * (rdx * qx + rdy * qy + rdz * qz);
+            if (vv < 0.0 || (u + vv) > 1.0) continue;
+            const double tt = f * (e2x * qx + e2y * qy + e2z * qz);
+            if (tt > EPS) {
+                const double hitZ = roz + tt * rdz;
+                if (hitZ > maxZ) maxZ = hitZ;
+            }
+        }
+        out = maxZ;
+    } while (false);
+    gh[t] = out;
+}
+
+// Mask of every cell, halo included (05:469-489 and the halo rule 05:509-531).
+__global__ __launch_bounds__(256) void k_mask_from_heights(VoxelArgs v, GridDesc g, const double *gh,
+                                                          uint8_t *mask) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    if (t >= total) return;
+    const int i = (int)(t % (g.nx + 2));
+    const int j = (int)((t / (g.nx + 2)) % (g.ny + 2));
+    const int k = (int)(t / ((long long)(g.nx + 2) * (g.ny + 2)));
+    const long long c = (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    uint8_t m;
+    if (k == 0) m = v.ground ? MK_SOLID : 0;
+    else if (k == g.nz + 1 || i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1) m = 0;
+    else {
+        const double z = v.z_min + ((k - 1) + 0.5) * v.dx;
+        const double h = gh[(long long)j * (g.nx + 2) + i];
+        m = (v.use_geometry && z < h) ? MK_SOLID : 0;
+        if (z > v.sky_z) m = 0;
+    }
+    mask[c] = m;
+}
+
+hipError_t launch_column_heights(const VoxelArgs &v, const GridDesc &g, double *gh, hipStream_t s) {
+    const long long ncol = (long long)(g.nx + 2) * (g.ny + 2);
+    k_column_heights<<<(unsigned)((ncol + 255) / 256), 256, 0, s>>>(v, g, gh);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_from_heights(const VoxelArgs &v, const GridDesc &g, const double *gh,
+                                    uint8_t *mask, hipStream_t s) {
+    const long long total = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    k_mask_from_heights<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(v, g, gh, mask);
+    return hipGetLastError();
+}
+
+// NEAR bit on fluid interior cells with a solid neighbour among the 18 links; keeps the
+// SOLID and PROBE bits.
+__global__ __launch_bounds__(256) void k_mask_flags(uint8_t *mask, GridDesc g) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    if (n >= total) return;
+    const int i = (int)(n % g.nx) + 1;
+    const int j = (int)((n / g.nx) % g.ny) + 1;
+    const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
+    const long long c = (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    uint8_t m = mask[c] & (uint8_t)~MK_NEAR;
+    if (!(m & MK_SOLID)) {
+        for (int q = 1; q < Q; ++q) {
+            const long long o = (long long)EX[q] + (long long)EY[q] * g.px + (long long)EZ[q] * g.plane;
+            if (mask[c + o] & MK_SOLID) { m |= MK_NEAR; break; }
+        }
+    }
+    mask[c] = m;
+}
+
+hipError_t launch_mask_flags(uint8_t *mask, const GridDesc &g, hipStream_t s) {
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    k_mask_flags<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(mask, g);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fill_f32(float *p, long long n, float v) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, long long n, uint8_t v) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t *p, long long n, uint8_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace nlos

@@ -1,0 +1,128 @@
+"""Seeded synthetic inputs: box cities as binary STL, moving sources, ground probes.
+
+The city follows the moving-source city workload: axis-aligned boxes with footprints
+uniform in [10, 40] m, heights lognormal(mu = ln 20, sigma = 0.5) capped at 200 m, added
+until 35 % of the ground is covered, a rigid ground plane, sky height above the tallest
+building.  Randomness comes from numpy's PCG64 generator with an explicit seed, so every
+machine builds the same city.  The STL layout is the one GeometryEngine::load reads
+(05_sensor_illusion.cpp:49-58, 70-111).
+"""
+from __future__ import annotations
+
+import math
+import struct
+from dataclasses import dataclass, field
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+import nlos_engine as nlos
+
+Box = Tuple[float, float, float, float, float, float]   # x0, x1, y0, y1, z0, z1
+
+
+def box_triangles(b: Box):
+    x0, x1, y0, y1, z0, z1 = b
+    p = [(x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0),
+         (x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)]
+    quads = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+    out = []
+    for a, b_, c, d in quads:
+        out.append((p[a], p[b_], p[c]))
+        out.append((p[a], p[c], p[d]))
+    return out
+
+
+def stl_bytes(boxes: Sequence[Box], header: bytes = b"seeded city") -> bytes:
+    tris = [t for b in boxes for t in box_triangles(b)]
+    buf = bytearray(header.ljust(80, b"\0")[:80])
+    buf += struct.pack("<I", len(tris))
+    rec = struct.Struct("<12fH")
+    for (a, b, c) in tris:
+        buf += rec.pack(0.0, 0.0, 0.0, *a, *b, *c, 0)
+    return bytes(buf)
+
+
+def seeded_city(width_m: float, depth_m: float, seed: int = 0, coverage: float = 0.35,
+                fp_lo: float = 10.0, fp_hi: float = 40.0, h_mu: float = math.log(20.0),
+                h_sigma: float = 0.5, h_cap: float = 200.0, margin_m: float = 0.0) -> List[Box]:
+    """Boxes placed uniformly over [margin, width-margin] x [margin, depth-margin] until
+    the covered ground fraction of the whole area reaches `coverage` (1 m raster)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    W, D = int(math.ceil(width_m)), int(math.ceil(depth_m))
+    cover = np.zeros((D, W), bool)
+    boxes: List[Box] = []
+    target = coverage * W * D
+    covered = 0
+    while covered < target:
+        fx, fy = rng.uniform(fp_lo, fp_hi, 2)
+        x0 = rng.uniform(margin_m, max(margin_m, width_m - margin_m - fx))
+        y0 = rng.uniform(margin_m, max(margin_m, depth_m - margin_m - fy))
+        h = min(h_cap, float(np.exp(rng.normal(h_mu, h_sigma))))
+        boxes.append((x0, x0 + fx, y0, y0 + fy, 0.0, h))
+        i0, i1 = int(x0), min(W, int(math.ceil(x0 + fx)))
+        j0, j1 = int(y0), min(D, int(math.ceil(y0 + fy)))
+        before = cover[j0:j1, i0:i1].sum()
+        cover[j0:j1, i0:i1] = True
+        covered += int(cover[j0:j1, i0:i1].sum() - before)
+    return boxes
+
+
+@dataclass
+class CityWorkload:
+    """The moving-source city scene on an nx x ny x nz grid of spacing dx."""
+    nx: int = 1024
+    ny: int = 1024
+    nz: int = 256
+    dx: float = 1.0
+    seed: int = 0
+    sponge_cells: int = 32
+    c_phys: float = 343.0
+    tau: float = 0.5 + 1e-2
+    freq: float = 200.0
+    amp_rho: float = 1e-4          # injected mass per step at the waveform peak, in rho0
+    height: float = 30.0
+    speed: float = 20.0
+    n_probes: int = 64
+    boxes: List[Box] = field(default_factory=list)
+
+    def config(self, precision="f16s") -> nlos.Config:
+        return nlos.default_config(
+            x_min=0.0, x_max=self.nx * self.dx, y_min=0.0, y_max=self.ny * self.dx,
+            z_min=0.0, z_max=self.nz * self.dx, dx=self.dx, c_phys=self.c_phys, tau=self.tau,
+            sponge_width=self.sponge_cells * self.dx, sponge_strength=0.1, sky_z=1e9,
+            precision=precision, world_offset=(0.0, 0.0, 0.0))
+
+    def build_boxes(self) -> List[Box]:
+        if not self.boxes:
+            self.boxes = seeded_city(self.nx * self.dx, self.ny * self.dx, seed=self.seed)
+        return self.boxes
+
+    def stl(self) -> bytes:
+        return stl_bytes(self.build_boxes())
+
+    def source(self) -> nlos.Source:
+        dt = nlos.dt_of(self.config())
+        x0, y0 = 0.30 * self.nx * self.dx, 0.45 * self.ny * self.dx
+        x1 = 0.70 * self.nx * self.dx
+        return nlos.default_source(sx=x0, sy=y0, sz=self.height, ex=x1, ey=y0, ez=self.height,
+                                   speed=self.speed, amp=self.amp_rho / dt, freq=self.freq,
+                                   waveform=nlos.WAVE_SINE, injection=nlos.INJECT_TRILINEAR)
+
+    def probes(self, heights: np.ndarray) -> np.ndarray:
+        """Ground-level probes: seeded columns within 120 m of the source start whose
+        ground cell is fluid, at the centre of the first cell above ground."""
+        rng = np.random.Generator(np.random.PCG64(self.seed + 1))
+        x0, y0 = 0.30 * self.nx * self.dx, 0.45 * self.ny * self.dx
+        pts = []
+        while len(pts) < self.n_probes:
+            r = 120.0 * math.sqrt(rng.uniform(0.0, 1.0))
+            a = rng.uniform(0.0, 2.0 * math.pi)
+            x, y = x0 + r * math.cos(a), y0 + r * math.sin(a)
+            i, j = int(x / self.dx), int(y / self.dx)
+            if not (0 <= i < self.nx and 0 <= j < self.ny):
+                continue
+            if heights[j + 1, i + 1] > 0.5 * self.dx:   # column (with halo offset) is built
+                continue
+            pts.append((x, y, 0.5 * self.dx))
+        return np.array(pts)

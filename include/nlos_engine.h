@@ -1,0 +1,204 @@
+/*
+ * nlos_engine.h - C ABI of the MI355X D3Q19 lattice-Boltzmann acoustic engine.
+ *
+ * Drop-in for the time-stepping path of the reference solver `LBMSolver`
+ * (05_sensor_illusion.cpp:241-1122; moving-source variant 08_city_scale_uav.cpp:540-628,
+ * 1026-1060).  The reference has no library API: its surface is the LBMSolver class, whose
+ * public fields (05:243-268) become `nlos_config` / `nlos_source` / probes, and whose public
+ * methods map one to one onto the entry points below:
+ *
+ *   LBMSolver::init               05:299-450   nlos_create + nlos_load_stl / nlos_set_mask
+ *                                              + nlos_set_probes
+ *   LBMSolver::apply_source       05:576-667   nlos_apply_source
+ *   LBMSolver::exchange_halos     05:671-904   nlos_exchange_halos
+ *   LBMSolver::step_collision_bgk 05:906-972   nlos_step_collision
+ *   LBMSolver::write_vtk          05:975-1062  nlos_read_field / nlos_write_vtk
+ *   LBMSolver::run                05:1064-1121 nlos_step (loop body 05:1077-1093)
+ *                                              + nlos_write_probe_files (05:1106-1118)
+ *
+ * Conventions: every call returns 0 (NLOS_OK) or an nlos_status; nlos_last_error() gives a
+ * message for the calling thread.  One host thread per engine.  Work is ordered on the
+ * engine's own HIP stream; calls that hand data back to the host synchronise.  Device
+ * memory is owned by the engine; every host array is owned by the caller and copied.
+ * Non-finite populations are flagged on the device and reported as NLOS_ERR_NONFINITE by
+ * the next synchronising call.
+ */
+#ifndef NLOS_ENGINE_H
+#define NLOS_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLOS_ABI_VERSION 1
+
+typedef struct nlos_engine nlos_engine;
+
+typedef enum nlos_status {
+    NLOS_OK = 0,
+    NLOS_ERR_ARG = 1,         /* bad argument or configuration */
+    NLOS_ERR_HIP = 2,         /* HIP runtime failure */
+    NLOS_ERR_STATE = 3,       /* call out of order */
+    NLOS_ERR_NONFINITE = 4,   /* NaN/Inf density seen during stepping */
+    NLOS_ERR_IO = 5,          /* file could not be read or written */
+    NLOS_ERR_NOMEM = 6,       /* device allocation failed */
+    NLOS_ERR_UNSUPPORTED = 7  /* option not available for this precision */
+} nlos_status;
+
+/* Population storage.  NLOS_F64 keeps full populations f in FP64 and follows the
+ * reference's arithmetic operation for operation.  NLOS_F32 and NLOS_F16S store the
+ * deviation g = f - w_q (rest density 1) in FP32 or in FP16 scaled by 2^f16_scale_log2, and
+ * do all collision arithmetic in FP32. */
+typedef enum nlos_precision { NLOS_F64 = 0, NLOS_F32 = 1, NLOS_F16S = 2 } nlos_precision;
+
+/* Domain faces, used for sponge and NSCBC face sets. */
+enum {
+    NLOS_FACE_XM = 1, NLOS_FACE_XP = 2, NLOS_FACE_YM = 4,
+    NLOS_FACE_YP = 8, NLOS_FACE_ZM = 16, NLOS_FACE_ZP = 32
+};
+/* The reference's set: every face but the ground (05:491-497 sponge, 05:684-835 NSCBC). */
+#define NLOS_FACES_REFERENCE (NLOS_FACE_XM | NLOS_FACE_XP | NLOS_FACE_YM | NLOS_FACE_YP | NLOS_FACE_ZP)
+
+typedef struct nlos_config {
+    /* Physical box and spacing; grid = (int)((max-min)/dx + 0.5) per axis (05:332-334). */
+    double x_min, x_max, y_min, y_max, z_min, z_max;   /* 05:250-252 */
+    double dx;                 /* 05:253 (the programs set 1.5/1.0/0.5 at 05:1129) */
+    double c_phys;             /* UnitConverter::c_phys, 340 (05:198); dt = cs*dx/c (05:200-203) */
+    double tau;                /* BGK relaxation time, 0.55 (05:268); must exceed 0.5 */
+    double sponge_width;       /* metres; 05:455 uses 20*dx */
+    double sponge_strength;    /* 0.35 (05:456); sigma = strength*((w-d)/w)^2 (05:498-502) */
+    double sky_z;              /* cells above are forced fluid, 120 (05:458, 489) */
+    double wall_reflect;       /* 1 = full half-way bounce-back (05:923-926); <1 damps it */
+    double world_offset[3];    /* added to the VTK origin when output_world_coords (05:293-296) */
+    int32_t sponge_faces;      /* NLOS_FACE_* set, reference NLOS_FACES_REFERENCE */
+    int32_t nscbc_faces;       /* NLOS_FACE_* set, reference NLOS_FACES_REFERENCE */
+    int32_t ground;            /* 1: bottom halo is solid (05:520-525); 0: open */
+    int32_t use_geometry;      /* 05:248 */
+    int32_t output_world_coords; /* 05:255 */
+    int32_t precision;         /* nlos_precision */
+    int32_t f16_scale_log2;    /* NLOS_F16S: stored value = g * 2^k; 0 selects the default 24 */
+    int32_t reserved[5];
+} nlos_config;
+
+typedef enum nlos_waveform {
+    NLOS_WAVE_GAUSS_PULSE = 0, /* amp*exp(-0.5*((t-t0)/sigma_t)^2)   05:579-581 */
+    NLOS_WAVE_SINE = 1,        /* amp*sin(2*pi*freq*t)               08:543 */
+    NLOS_WAVE_SAMPLED = 2,     /* amp*samples[s] at step s (s = round(t/dt)) */
+    NLOS_WAVE_RICKER = 3       /* amp*(1-2a)exp(-a), a=(pi*freq*(t-t0))^2 */
+} nlos_waveform;
+
+typedef enum nlos_injection {
+    NLOS_INJECT_GAUSS5 = 0,    /* normalised Gaussian stencil, R=2, sigma=1.25 cells (05:587-665) */
+    NLOS_INJECT_TRILINEAR = 1  /* 8 cell-centred trilinear weights, normalised over fluid cells */
+} nlos_injection;
+
+typedef struct nlos_source {
+    double sx, sy, sz;         /* DroneSource start (05:213) */
+    double ex, ey, ez;         /* DroneSource end (05:214) */
+    double speed;              /* m/s; <1e-12 keeps the source at the start (05:225) */
+    double amp;                /* 05:216 */
+    double freq;               /* Hz (08:543) */
+    double t0, sigma_t;        /* pulse centre/width, 0.05 / 0.005 (05:579-580) */
+    double gain;               /* source_gain, 1.0 (05:583) */
+    double stencil_sigma_cells;/* 1.25 (05:588) */
+    int32_t stencil_radius;    /* 2 (05:587) */
+    int32_t waveform;          /* nlos_waveform */
+    int32_t injection;         /* nlos_injection */
+    int32_t n_samples;         /* NLOS_WAVE_SAMPLED: number of samples */
+    const double *samples;     /* NLOS_WAVE_SAMPLED: copied by nlos_set_source */
+} nlos_source;
+
+typedef struct nlos_info {
+    int32_t nx, ny, nz;        /* interior cells NXg, NYg, NZg (05:332-334) */
+    int32_t precision;
+    double dt;                 /* seconds (05:200-203) */
+    int64_t step;              /* next time index s; t = s*dt (05:1079) */
+    int64_t n_probes;
+    int64_t n_solid_boundary;  /* solid cells on NSCBC faces (their state is tracked) */
+    double bytes_per_update;   /* algorithmic HBM bytes per lattice update of the step kernel */
+    size_t device_bytes;       /* device memory held by the engine */
+} nlos_info;
+
+typedef struct nlos_timing {
+    int64_t launches;          /* step-kernel launches timed since nlos_timing_reset */
+    double total_ms;           /* summed step-kernel durations (HIP events on the engine stream) */
+    double avg_ms;
+} nlos_timing;
+
+/* ---- lifetime ---------------------------------------------------------------------- */
+int nlos_abi_version(void);
+const char *nlos_last_error(void);
+/* Fills the reference defaults of 05:243-268, 455-458, 669 (DX=2, box 1650x1150x250 m). */
+int nlos_config_default(nlos_config *cfg);
+/* Fills the reference defaults of 05:579-589 (pulse t0=0.05, sigma_t=0.005, R=2, 1.25). */
+int nlos_source_default(nlos_source *src);
+/* LBMSolver::init (05:299-395): grid, tables, f = w everywhere, empty geometry. */
+int nlos_create(const nlos_config *cfg, nlos_engine **out);
+int nlos_destroy(nlos_engine *e);
+int nlos_get_info(const nlos_engine *e, nlos_info *info);
+
+/* ---- geometry ------------------------------------------------------------------------ */
+/* GeometryEngine::load + buildGrid (05:70-142) on the host, then the column top-hit ray
+ * kernel (getGroundHeight, 05:144-192) and the mask rule of init_domain (05:453-532) on
+ * the GPU.  A file that cannot be read returns NLOS_ERR_IO and leaves empty geometry, as
+ * the reference warns and continues (05:407-410). */
+int nlos_load_stl(nlos_engine *e, const char *path, double scale, double sx, double sy, double sz);
+int nlos_voxelize_stl(nlos_engine *e, const void *stl, size_t len,
+                      double scale, double sx, double sy, double sz);
+/* Column heights gH of the last voxelisation over (nx+2)*(ny+2) columns incl. halo. */
+int nlos_get_ground_heights(nlos_engine *e, double *out);
+/* Interior solid mask, nx*ny*nz bytes, x fastest, 0 = FLUID, 1 = SOLID (05:36). */
+int nlos_set_mask(nlos_engine *e, const uint8_t *mask);
+int nlos_get_mask(nlos_engine *e, uint8_t *mask);
+
+/* ---- initial state, source, probes --------------------------------------------------- */
+/* Initial populations f = feq(rho, 0) = w_q*rho on fluid cells (interior rho, x fastest). */
+int nlos_set_density(nlos_engine *e, const double *rho);
+int nlos_set_source(nlos_engine *e, const nlos_source *src);
+/* Probe positions (x,y,z triples, metres); cell = floor((p-min)/dx) (05:417-419).  A probe
+ * outside the grid is NLOS_ERR_ARG; one inside a solid is accepted and records 0 (rho=1,
+ * 05:936) - its flag is set in in_solid (may be NULL). */
+int nlos_set_probes(nlos_engine *e, const double *xyz, int32_t n, uint8_t *in_solid);
+
+/* ---- the time step -------------------------------------------------------------------- */
+/* The three phases of one reference step, for callers that drive them one by one. */
+int nlos_apply_source(nlos_engine *e, double t);   /* 05:576-667 */
+int nlos_exchange_halos(nlos_engine *e);           /* 05:671-904 (single domain: ghosts only) */
+int nlos_step_collision(nlos_engine *e);           /* 05:906-972 + probe record 05:1085-1093 */
+/* n fused steps s = step .. step+n-1, each apply_source(s*dt), exchange_halos,
+ * step_collision, probe record.  flags: NLOS_STEP_CAPTURE_FIELD keeps rho-1 of the last
+ * step for nlos_read_field. */
+#define NLOS_STEP_CAPTURE_FIELD 1
+int nlos_step(nlos_engine *e, int64_t n, int32_t flags);
+int nlos_synchronize(nlos_engine *e);
+
+/* ---- results -------------------------------------------------------------------------- */
+/* Probe histories rho-1, probe-major [probe][step], steps recorded so far (at most
+ * max_steps per probe).  *n_steps receives the number recorded. */
+int nlos_read_probes(nlos_engine *e, double *out, int64_t max_steps, int64_t *n_steps);
+/* float(rho-1) of the last captured step over the interior, x fastest (05:979-985). */
+int nlos_read_field(nlos_engine *e, float *out);
+/* probe_<name>.txt, "%.8e" one value per line (05:1110-1114). */
+int nlos_write_probe_files(nlos_engine *e, const char *dir, const char *const *names);
+/* Legacy ASCII VTK of the captured field, byte-compatible with 05:1045-1053. */
+int nlos_write_vtk(nlos_engine *e, const char *path);
+
+/* ---- checkpoint / restore ------------------------------------------------------------ */
+/* Serialises populations, tracked boundary-solid state, step index and probe histories.
+ * With buf == NULL only *needed is set. */
+int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *needed);
+int nlos_restore(nlos_engine *e, const void *buf, size_t len);
+
+/* ---- measurement ------------------------------------------------------------------------ */
+/* When on, every step-kernel launch is bracketed by HIP events on the engine stream. */
+int nlos_timing_enable(nlos_engine *e, int32_t on);
+int nlos_timing_reset(nlos_engine *e);
+int nlos_timing_get(nlos_engine *e, nlos_timing *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NLOS_ENGINE_H */

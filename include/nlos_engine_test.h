@@ -1,0 +1,26 @@
+/*
+ * nlos_engine_test.h - test hooks exported by the engine library beside the drop-in ABI of
+ * nlos_engine.h.  They have no counterpart in the reference; parity tests use them to read
+ * the full population state and to force a non-finite value from the host.
+ */
+#ifndef NLOS_ENGINE_TEST_H
+#define NLOS_ENGINE_TEST_H
+
+#include "nlos_engine.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Post-collision populations f* of every interior cell as FP64 full populations,
+ * array-of-structures [cell][q] over the interior (x fastest); NaN on solid cells. */
+int nlos_debug_read_f(nlos_engine *e, double *out);
+
+/* Overwrite the stored population q of interior cell (i,j,k) (0-based) with the full
+ * population value f, converted to the engine's storage form. */
+int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, int32_t q, double f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NLOS_ENGINE_TEST_H */
