@@ -1,0 +1,186 @@
+"""Benchmark of the D3Q19 acoustic time step on one MI355X.
+
+Workload: the moving-source city grid, 1024 x 1024 x 256 cells at dx = 1 m (c = 343 m/s,
+dt = dx/(sqrt(3)*343)), seeded box city (footprints 10-40 m, lognormal heights capped at
+200 m, 35 % coverage) voxelised on the GPU from its STL, rigid ground, 32-cell sponge on
+five faces, a 200 Hz source moving at 20 m/s 30 m above ground with trilinear injection,
+64 ground-level probes.  A step is one full time step (source, open-face ghosts,
+collide-stream sweep, probe record) over the whole grid.
+
+The headline value is FP16-store (FP32 arithmetic) MLUPS, counted over all cells as in
+04_hpc_scalability.cpp:399-401.  The same run times the FP32 and FP64 paths, reports the
+probe error of FP16/FP32 against FP64 over the same steps, the step kernel's HBM roofline
+fraction (HIP events on the engine stream), and the CPU baseline (the oracle, FP64
+two-array OpenMP restatement of the reference) on a bounded sub-box of the same city.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "city-scale-nlos-acoustic-engine-code_amd"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+import nlos_engine as nlos  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS & % of HBM roofline, city grid: FP16-store vs FP64 at stated probe error"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_precision(wl, stl, precision, warmup, steps, timing=True):
+    cfg = wl.config(precision)
+    e = nlos.Engine(cfg)
+    e.voxelize_stl(stl)
+    heights = e.ground_heights()
+    probes = wl.probes(heights)
+    e.set_probes(probes)
+    e.set_source(wl.source())
+    e.step(warmup)
+    e.synchronize()
+    if timing:
+        e.timing(True)
+        e.timing_reset()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    tm = e.timing(False) if timing else None
+    info = e.info()
+    mask = e.get_mask()
+    trace = e.read_probes()
+    e.close()
+    cells = info.nx * info.ny * info.nz
+    solid = int(mask.sum())
+    return {
+        "precision": precision, "wall_s": wall, "cells": cells, "solid": solid,
+        "mlups": cells * steps / wall / 1e6, "ms_per_step": 1e3 * wall / steps,
+        "kernel_avg_ms": tm.avg_ms if tm else None, "kernel_launches": tm.launches if tm else 0,
+        "bytes_per_update": info.bytes_per_update, "probes": trace, "mask": mask,
+    }
+
+
+def cpu_baseline(mask_full, wl, budget_s=15.0, sub=(192, 192, 128)):
+    """The oracle on a sub-box of the same city (corner at the source start), FP64
+    two-array OpenMP over the host cores; MLUPS over all sub-box cells."""
+    from oracle_ffi import Oracle
+    nx, ny, nz = sub
+    i0, j0 = int(0.30 * wl.nx) - nx // 2, int(0.45 * wl.ny) - ny // 2
+    m = np.ascontiguousarray(mask_full[:nz, j0:j0 + ny, i0:i0 + nx])
+    cfg = nlos.default_config(x_min=0.0, x_max=float(nx), y_min=0.0, y_max=float(ny), z_min=0.0,
+                              z_max=float(nz), dx=1.0, c_phys=wl.c_phys, tau=wl.tau,
+                              sponge_width=wl.sponge_cells, sponge_strength=0.1, sky_z=1e9)
+    o = Oracle(cfg)
+    o.set_mask(m)
+    src = wl.source()
+    src.sx, src.sy, src.ex, src.ey = nx / 2, ny / 2, nx / 2 + 100.0, ny / 2
+    o.set_source(src)
+    o.run(2)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.run(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 400:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    return {"value": nx * ny * nz * steps / el / 1e6, "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FP64 two-array OpenMP, {nx}x{ny}x{nz} sub-box of the same city, "
+                      f"{steps} steps in {el:.1f} s after 2 warm-up steps"}
+
+
+def rel_l2(a, b):
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a - b))
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="f16s,f32,f64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the single-domain path has no multi-GPU sharding)", file=sys.stderr)
+        sys.exit(2)
+    if not os.path.exists(nlos.LIB_PATH):
+        raise SystemExit("engine library not built; run __graft_entry__.build()")
+
+    wl = workloads.CityWorkload(nx=args.nx, ny=args.ny, nz=args.nz)
+    t = time.perf_counter()
+    stl = wl.stl()
+    log(f"city: {len(wl.boxes)} boxes, {len(stl)} STL bytes ({time.perf_counter() - t:.1f} s)")
+
+    res = {}
+    for p in args.only.split(","):
+        log(f"running {p}: warmup {args.warmup}, timed {args.steps}")
+        res[p] = run_precision(wl, stl, p, args.warmup, args.steps)
+        r = res[p]
+        log(f"  {p}: {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, "
+            f"kernel {r['kernel_avg_ms']:.3f} ms")
+
+    head = res.get("f16s") or next(iter(res.values()))
+    mask = head["mask"]
+    fluid = head["cells"] - head["solid"]
+    bpu = head["bytes_per_update"]
+    alg_bytes = fluid * bpu + head["solid"] * 1.0     # per sweep: 19 loads + 19 stores + mask
+    achieved = alg_bytes / (head["kernel_avg_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "storage": head["precision"],
+        "data": "synthetic",
+        "config": {"workload": f"moving-source city {args.nx}x{args.ny}x{args.nz}, dx=1 m, "
+                               f"{len(wl.boxes)} seeded boxes, 64 ground probes, "
+                               f"FP16-scaled deviation storage",
+                   "cells": head["cells"], "solid_cells": head["solid"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_update": bpu, "kernel_avg_ms": round(head["kernel_avg_ms"], 4)},
+    }
+    for p, r in res.items():
+        out[f"{p}_mlups"] = round(r["mlups"], 1)
+        out[f"{p}_kernel_ms"] = round(r["kernel_avg_ms"], 4)
+    if "f64" in res:
+        ref = res["f64"]["probes"]
+        for p in res:
+            if p != "f64":
+                out[f"probe_rel_l2_{p}_vs_f64"] = rel_l2(res[p]["probes"], ref)
+        if "f16s" in res:
+            out["speedup_f16s_vs_f64"] = round(res["f16s"]["mlups"] / res["f64"]["mlups"], 3)
+        out["probe_signal_max"] = float(np.abs(ref).max())
+        out["probe_steps"] = args.warmup + args.steps
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(mask, wl)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,71 @@
+"""Small seeded scenes shared by the oracle and GPU parity tests."""
+from __future__ import annotations
+
+import numpy as np
+
+import nlos_engine as nlos
+
+
+def box_scene(precision="f64", nx=40, ny=32, nz=24, seed=3):
+    """Buildings that cross the x-, y+ and top faces, a moving sine source, six probes
+    (one inside a solid), reference sponge/NSCBC face sets and ground."""
+    cfg = nlos.default_config(dx=1.0, x_min=0.0, x_max=float(nx), y_min=0.0, y_max=float(ny),
+                              z_min=0.0, z_max=float(nz), sponge_width=6.0, sponge_strength=0.35,
+                              tau=0.55, sky_z=1e9, precision=precision)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[:8, 5:11, 0:6] = 1            # crosses x-
+    solid[:, 26:32, 30:36] = 1          # crosses y+ and reaches the top plane
+    solid[:12, 12:18, 16:22] = 1
+    solid[:5, 2:6, 34:40] = 1           # crosses x+
+    rng = np.random.default_rng(seed)
+    for _ in range(6):
+        i, j = rng.integers(0, nx - 3), rng.integers(0, ny - 3)
+        solid[: rng.integers(1, 6), j:j + 3, i:i + 3] = 1
+    src = nlos.default_source(sx=14.3, sy=7.7, sz=9.2, ex=25.1, ey=20.4, ez=11.0, speed=60.0,
+                              amp=2e-2, freq=120.0, waveform=nlos.WAVE_SINE)
+    probes = [(3.5, 20.5, 1.5), (37.2, 15.3, 7.7), (20.0, 16.0, 20.9), (8.5, 8.5, 3.5),
+              (33.5, 29.5, 10.5), (1.2, 30.8, 22.5)]
+    return cfg, solid, src, probes
+
+
+def pulse_scene(precision="f64", n=28, tau=0.5 + 1e-4):
+    """Config-1 shape at small size: open bottom, sponge on all six faces, NSCBC on all six
+    faces, a cube scatterer, an initial Gaussian density pulse and no source."""
+    cfg = nlos.default_config(dx=1.0, x_min=0.0, x_max=float(n), y_min=0.0, y_max=float(n),
+                              z_min=0.0, z_max=float(n), sponge_width=4.0, sponge_strength=0.1,
+                              tau=tau, sky_z=1e9, ground=0, sponge_faces=nlos.FACES_ALL,
+                              nscbc_faces=nlos.FACES_ALL, precision=precision)
+    solid = np.zeros((n, n, n), np.uint8)
+    a, b = n // 2 - 3, n // 2 + 3
+    solid[a:b, a:b, a:b] = 1
+    z, y, x = np.meshgrid(np.arange(n) + 0.5, np.arange(n) + 0.5, np.arange(n) + 0.5, indexing="ij")
+    c = (n / 2 + 7.0, n / 2, n / 2)
+    r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+    rho = 1.0 + 1e-3 * np.exp(-r2 / (2 * 2.0 ** 2))
+    probes = [(4.5, 4.5, 4.5), (n - 5.5, n / 2, n / 2), (n / 2, 5.5, n / 2), (n / 2, n / 2, n - 5.5)]
+    return cfg, solid, rho, probes
+
+
+def reflect_scene(precision="f64", nx=30, ny=26, nz=20):
+    """Partially reflecting walls, trilinear injection of a Ricker wavelet, NSCBC off on
+    one face, a sponge on four faces."""
+    cfg = nlos.default_config(dx=1.0, x_min=0.0, x_max=float(nx), y_min=0.0, y_max=float(ny),
+                              z_min=0.0, z_max=float(nz), sponge_width=5.0, sponge_strength=0.2,
+                              tau=0.6, sky_z=1e9, wall_reflect=0.8,
+                              sponge_faces=nlos.FACE_XM | nlos.FACE_XP | nlos.FACE_YM | nlos.FACE_ZP,
+                              nscbc_faces=nlos.FACE_XM | nlos.FACE_XP | nlos.FACE_YP | nlos.FACE_ZP,
+                              precision=precision)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[:7, 8:14, 10:16] = 1
+    solid[:3, 0:4, 0:30] = 1
+    src = nlos.default_source(sx=6.6, sy=16.2, sz=4.4, ex=6.6, ey=16.2, ez=4.4, amp=5e-2,
+                              freq=90.0, t0=0.012, waveform=nlos.WAVE_RICKER,
+                              injection=nlos.INJECT_TRILINEAR)
+    probes = [(20.5, 20.5, 1.5), (25.0, 5.0, 4.0), (12.5, 11.5, 8.5)]
+    return cfg, solid, src, probes
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))

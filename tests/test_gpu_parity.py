@@ -1,0 +1,249 @@
+"""GPU parity of the HIP engine against the oracle and the compiled reference's goldens.
+
+FP64 (full populations, reference operation order) must be bit-identical to the oracle
+and reproduce the reference's probe files byte for byte.  FP32 and FP16S (deviation
+storage, FP32 arithmetic) are held to the relative-L2 tolerances written in each test.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import nlos_engine as nlos
+import scenes
+from oracle_ffi import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+pytestmark = pytest.mark.gpu
+
+# Reduced-precision tolerances on probe traces and fields, relative L2 against FP64.
+TOL_F32 = 1e-4
+TOL_F16 = 3e-2
+
+
+def _engine_run(cfg, solid, src, probes, steps, rho=None):
+    e = nlos.Engine(cfg)
+    e.set_mask(solid)
+    if rho is not None:
+        e.set_density(rho)
+    if src is not None:
+        e.set_source(src)
+    e.set_probes(probes)
+    e.step(steps, capture_field=True)
+    e.synchronize()
+    return e
+
+
+def _oracle_run(cfg, solid, src, probes, steps, rho=None):
+    o = Oracle(cfg)
+    o.set_mask(solid)
+    if rho is not None:
+        o.set_density(rho)
+    if src is not None:
+        o.set_source(src)
+    o.set_probes(probes)
+    o.run(steps)
+    return o
+
+
+def _oracle_interior_f(o):
+    return o.read_f()[1:-1, 1:-1, 1:-1, :]
+
+
+@pytest.mark.parametrize("scene", ["box", "pulse", "reflect"])
+def test_f64_bit_identical_to_oracle(gpu, scene):
+    print(f"[f64 vs oracle] scene={scene}", flush=True)
+    if scene == "box":
+        cfg, solid, src, probes = scenes.box_scene("f64")
+        rho, steps = None, 60
+    elif scene == "pulse":
+        cfg, solid, rho, probes = scenes.pulse_scene("f64")
+        src, steps = None, 40
+    else:
+        cfg, solid, src, probes = scenes.reflect_scene("f64")
+        rho, steps = None, 50
+    e = _engine_run(cfg, solid, src, probes, steps, rho)
+    o = _oracle_run(cfg, solid, src, probes, steps, rho)
+    got, want = e.read_probes(), o.read_probes()
+    assert got.shape == want.shape
+    assert np.abs(want).max() > 0.0
+    assert np.array_equal(got, want), np.abs(got - want).max()
+    assert np.array_equal(e.read_field(), o.read_field())
+    fe, fo = e.debug_read_f(), _oracle_interior_f(o)
+    fluid = solid == 0
+    assert np.array_equal(fe[fluid], fo[fluid])
+    e.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", TOL_F32), ("f16s", TOL_F16)])
+@pytest.mark.parametrize("scene", ["box", "reflect"])
+def test_reduced_precision_vs_f64(gpu, precision, tol, scene):
+    print(f"[{precision} vs f64] scene={scene}", flush=True)
+    make = scenes.box_scene if scene == "box" else scenes.reflect_scene
+    cfg, solid, src, probes = make(precision)
+    e = _engine_run(cfg, solid, src, probes, 80)
+    cfg64, _, _, _ = make("f64")
+    o = _oracle_run(cfg64, solid, src, probes, 80)
+    err = scenes.rel_l2(e.read_probes(), o.read_probes())
+    ferr = scenes.rel_l2(e.read_field(), o.read_field())
+    print(f"  probe rel-L2 {err:.3e}  field rel-L2 {ferr:.3e}")
+    assert err <= tol and ferr <= tol
+    e.close()
+
+
+def test_three_phase_api_equals_fused_step(gpu):
+    cfg, solid, src, probes = scenes.box_scene("f64")
+    a = _engine_run(cfg, solid, src, probes, 25)
+    b = nlos.Engine(cfg)
+    b.set_mask(solid)
+    b.set_source(src)
+    b.set_probes(probes)
+    dt = b.dt
+    for s in range(25):
+        b.apply_source(s * dt)
+        b.exchange_halos()
+        b.step_collision_bgk()
+    b.synchronize()
+    assert np.array_equal(a.read_probes(), b.read_probes())
+    assert np.array_equal(a.debug_read_f()[solid == 0], b.debug_read_f()[solid == 0])
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16s"])
+def test_rest_state_is_exact_fixed_point(gpu, precision):
+    cfg, solid, _, probes = scenes.box_scene(precision)
+    e = nlos.Engine(cfg)
+    e.set_mask(solid)
+    e.set_probes(probes)
+    e.step(31, capture_field=True)
+    e.synchronize()
+    assert not np.any(e.read_probes())
+    assert not np.any(e.read_field())
+    f = e.debug_read_f()[solid == 0]
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    assert np.array_equal(f, np.broadcast_to(w, f.shape))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f16s"])
+def test_checkpoint_restore_is_bitwise(gpu, precision):
+    cfg, solid, src, probes = scenes.box_scene(precision)
+    e = _engine_run(cfg, solid, src, probes, 17)
+    blob = e.checkpoint()
+    e.step(20)
+    e.synchronize()
+    p1, f1 = e.read_probes(), e.debug_read_f()
+    e.close()
+    r = nlos.Engine(cfg)
+    r.set_mask(solid)
+    r.set_source(src)
+    r.set_probes(probes)
+    r.restore(blob)
+    r.step(20)
+    r.synchronize()
+    assert np.array_equal(r.read_probes(), p1)
+    assert np.array_equal(r.debug_read_f(), f1, equal_nan=True)
+
+
+def test_nonfinite_population_is_reported(gpu):
+    cfg, solid, src, probes = scenes.box_scene("f32")
+    e = _engine_run(cfg, solid, src, probes, 3)
+    e.debug_poke(20, 3, 20, 7, float("nan"))
+    e.step(2)
+    with pytest.raises(nlos.EngineError) as ex:
+        e.synchronize()
+    assert ex.value.code == 4
+
+
+def test_voxelizer_matches_oracle_bitwise(gpu):
+    import city_stl
+    cfg = nlos.reference05_config()
+    stl = city_stl.reference_city_stl()
+    e = nlos.Engine(cfg)
+    e.voxelize_stl(stl, 1.0, -22248.0, -4123.0, -14.0)
+    o = Oracle(cfg)
+    assert o.load_stl(stl, 1.0, -22248.0, -4123.0, -14.0) == 0
+    assert np.array_equal(e.ground_heights(), o.ground_heights())
+    m = e.get_mask()
+    assert np.array_equal(m, o.get_mask())
+    assert m.sum() > 0
+    assert e.info().n_solid_boundary > 0
+
+
+def _reference05_engine(scene):
+    import city_stl
+    e = nlos.Engine(nlos.reference05_config())
+    if scene == "city":
+        e.voxelize_stl(city_stl.reference_city_stl(), 1.0, -22248.0, -4123.0, -14.0)
+    else:
+        assert e.load_stl("/nonexistent/csx_model_binary.stl") == 5   # empty geometry, 05:407-410
+    names, pts = nlos.reference05_probes()
+    e.set_probes(pts, names)
+    e.set_source(nlos.reference05_source())
+    return e
+
+
+@pytest.mark.parametrize("scene", ["flat", "city"])
+def test_f64_reproduces_reference_probe_files(gpu, scene, tmp_path):
+    """05_sensor_illusion's own run (393+1 steps) compiled from the reference sources."""
+    print(f"[reference goldens] scene={scene}", flush=True)
+    e = _reference05_engine(scene)
+    frames = json.load(open(os.path.join(GOLDEN, f"ref05_{scene}", "vtk_frames.json")))
+    total = int(round(1.0 / e.dt))
+    assert total == 393
+    done = 0
+    for fr in ("50", "98"):
+        s = int(fr) * 4                      # out_interval = 393 // 80 = 4
+        e.step(s + 1 - done, capture_field=True)
+        done = s + 1
+        e.write_vtk(str(tmp_path / "f.vtk"))
+        lines = open(tmp_path / "f.vtk").read().split("\n")
+        want = frames["frames"][fr]
+        assert lines[:10] == want["header"]
+        vals = lines[10:]
+        nx, ny, nz = frames["grid"]
+        st = want["stride"]
+        for k, exp in want["slices"].items():
+            k = int(k)
+            got = [vals[(k * ny + j) * nx + i] for j in range(0, ny, st) for i in range(0, nx, st)]
+            assert got == exp, f"frame {fr} slice {k}"
+    e.step(total + 1 - done)
+    e.write_probe_files(str(tmp_path))
+    for n in ("B1", "B2", "C1", "C2"):
+        got = open(tmp_path / f"probe_{n}.txt").read()
+        want = open(os.path.join(GOLDEN, f"ref05_{scene}", f"probe_{n}.txt")).read()
+        assert got == want, n
+    e.close()
+
+
+def test_run_loop_frames(gpu, tmp_path):
+    """Engine.run mirrors LBMSolver::run's frame cadence (05:1064-1097)."""
+    cfg, solid, src, probes = scenes.box_scene("f32")
+    e = nlos.Engine(cfg)
+    e.set_mask(solid)
+    e.set_source(src)
+    e.set_probes(probes, [f"P{i}" for i in range(len(probes))])
+    total = e.run(40 * e.dt, 5, vtk_dir=str(tmp_path / "vtk"), probe_dir=str(tmp_path))
+    assert total == 40
+    assert len(os.listdir(tmp_path / "vtk")) == 40 // (40 // 5) + 1
+    assert len(open(tmp_path / "probe_P0.txt").read().split()) == 41
+
+
+@pytest.mark.parametrize("scene", ["city", "flat"])
+def test_cpp_lbmsolver_program_reproduces_reference(gpu, scene, tmp_path):
+    """host/sensor_illusion.cpp: 05's main on the C++ LBMSolver surface (lbm_solver.hpp),
+    run like the reference in a directory holding csx_model_binary.stl."""
+    import subprocess
+    import city_stl
+    exe = os.path.join(os.path.dirname(nlos.LIB_PATH), "sensor_illusion")
+    assert os.path.exists(exe), "build the package (make) first"
+    if scene == "city":
+        (tmp_path / "csx_model_binary.stl").write_bytes(city_stl.reference_city_stl())
+    out = subprocess.run([exe, "--no-vtk"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    for n in ("B1", "B2", "C1", "C2"):
+        got = (tmp_path / f"probe_{n}.txt").read_text()
+        want = open(os.path.join(GOLDEN, f"ref05_{scene}", f"probe_{n}.txt")).read()
+        assert got == want, n
