@@ -115,6 +115,8 @@ def main() -> None:
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="f16s,f32,f64")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy kernel (PMC byte calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -176,6 +178,14 @@ def main() -> None:
             out["speedup_f16s_vs_f64"] = round(res["f16s"]["mlups"] / res["f64"]["mlups"], 3)
         out["probe_signal_max"] = float(np.abs(ref).max())
         out["probe_steps"] = args.warmup + args.steps
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30}
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(mask, wl)
