@@ -38,8 +38,12 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_precision(wl, stl, precision, warmup, steps, timing=True):
+PATTERN_NAMES = {1: "inplace A-A", 2: "two-array pull"}
+
+
+def run_precision(wl, stl, precision, warmup, steps, timing=True, pattern="auto"):
     cfg = wl.config(precision)
+    cfg.stream_pattern = nlos.PATTERNS[pattern]
     e = nlos.Engine(cfg)
     e.voxelize_stl(stl)
     heights = e.ground_heights()
@@ -67,6 +71,7 @@ def run_precision(wl, stl, precision, warmup, steps, timing=True):
         "mlups": cells * steps / wall / 1e6, "ms_per_step": 1e3 * wall / steps,
         "kernel_avg_ms": tm.avg_ms if tm else None, "kernel_launches": tm.launches if tm else 0,
         "bytes_per_update": info.bytes_per_update, "probes": trace, "mask": mask,
+        "pattern": PATTERN_NAMES.get(info.stream_pattern, str(info.stream_pattern)),
     }
 
 
@@ -114,7 +119,10 @@ def main() -> None:
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=600,
+                    help="steps of the FP16S/FP32 vs FP64 probe-error run (0 to skip)")
     ap.add_argument("--only", default="f16s,f32,f64")
+    ap.add_argument("--pattern", default="auto", choices=["auto", "inplace", "two_array"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy kernel (PMC byte calibration)")
     args = ap.parse_args()
@@ -133,10 +141,10 @@ def main() -> None:
     res = {}
     for p in args.only.split(","):
         log(f"running {p}: warmup {args.warmup}, timed {args.steps}")
-        res[p] = run_precision(wl, stl, p, args.warmup, args.steps)
+        res[p] = run_precision(wl, stl, p, args.warmup, args.steps, pattern=args.pattern)
         r = res[p]
         log(f"  {p}: {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, "
-            f"kernel {r['kernel_avg_ms']:.3f} ms")
+            f"kernel {r['kernel_avg_ms']:.3f} ms ({r['pattern']})")
 
     head = res.get("f16s") or next(iter(res.values()))
     mask = head["mask"]
@@ -161,7 +169,8 @@ def main() -> None:
         "config": {"workload": f"moving-source city {args.nx}x{args.ny}x{args.nz}, dx=1 m, "
                                f"{len(wl.boxes)} seeded boxes, 64 ground probes, "
                                f"FP16-scaled deviation storage",
-                   "cells": head["cells"], "solid_cells": head["solid"]},
+                   "cells": head["cells"], "solid_cells": head["solid"],
+                   "stream_pattern": head["pattern"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_update": bpu, "kernel_avg_ms": round(head["kernel_avg_ms"], 4)},
@@ -169,23 +178,30 @@ def main() -> None:
     for p, r in res.items():
         out[f"{p}_mlups"] = round(r["mlups"], 1)
         out[f"{p}_kernel_ms"] = round(r["kernel_avg_ms"], 4)
-    if "f64" in res:
-        ref = res["f64"]["probes"]
+    if "f64" in res and "f16s" in res:
+        out["speedup_f16s_vs_f64"] = round(res["f16s"]["mlups"] / res["f64"]["mlups"], 3)
+    if args.error_steps > 0 and "f64" in res:
+        # Probe error of the reduced-precision paths against FP64 over the same steps
+        # (relative L2 over all 64 probe traces, and the worst single probe).
+        log(f"probe error run: {args.error_steps} steps per precision")
+        traces = {}
         for p in res:
-            if p != "f64":
-                out[f"probe_rel_l2_{p}_vs_f64"] = rel_l2(res[p]["probes"], ref)
-        if "f16s" in res:
-            out["speedup_f16s_vs_f64"] = round(res["f16s"]["mlups"] / res["f64"]["mlups"], 3)
-        out["probe_signal_max"] = float(np.abs(ref).max())
-        out["probe_steps"] = args.warmup + args.steps
+            traces[p] = run_precision(wl, stl, p, 0, args.error_steps, timing=False,
+                                      pattern=args.pattern)["probes"]
+        ref = traces["f64"]
+        err = {"steps": args.error_steps, "signal_max": float(np.abs(ref).max())}
+        for p in traces:
+            if p == "f64":
+                continue
+            per = [rel_l2(a, b) for a, b in zip(traces[p], ref) if np.abs(b).max() > 0]
+            err[p] = {"rel_l2": rel_l2(traces[p], ref), "worst_probe_rel_l2": max(per) if per else None}
+        out["probe_error_vs_f64"] = err
     if args.calibrate_copy:
-        import torch
-        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30}
+        # A plain copy kernel (reads 2 GiB, writes 2 GiB per launch): its PMC byte counts
+        # calibrate FETCH_SIZE/WRITE_SIZE, its time gives the practical copy rate.
+        ms = nlos.copy_kernel_ms(1 << 31, 5)
+        out["copy_kernel"] = {"bytes_read": 1 << 31, "bytes_written": 1 << 31, "avg_ms": round(ms, 4),
+                              "gbs": round(2 * (1 << 31) / (ms * 1e-3) / 1e9, 1)}
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(mask, wl)

@@ -2,6 +2,8 @@
 // kernel argument blocks.  Included by host and device translation units.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace nlos {
@@ -39,24 +41,37 @@ constexpr uint8_t MK_PROBE = 4;   // fluid or solid cell that carries a probe
 // Face bits as in nlos_engine.h.
 constexpr int F_XM = 1, F_XP = 2, F_YM = 4, F_YP = 8, F_ZM = 16, F_ZP = 32;
 
-// Grid: interior nx*ny*nz, one halo layer, x pitch px >= nx+2.  Cell (i,j,k) with i in
-// [0,nx+1] etc. sits at i + px*(j + py*k); population q of it at q*ncell + cell.
+// Grid: interior nx*ny*nz, one halo layer, x pitch px >= nx+2 (a multiple of 64).  Cell
+// (i,j,k) with i in [0,nx+1] etc. sits at origin + i + px*(j + py*k); population q of it
+// at q*ncell + cell.  origin = 63 puts every interior row start (i = 1) on a 64-element
+// boundary, so a wavefront's 64 consecutive cells fill whole cache lines.
 struct GridDesc {
     int nx, ny, nz;
     int px, py, pz;
     long long plane;   // px*py
-    long long ncell;   // px*py*pz
+    long long ncell;   // population stride, a multiple of 64
+    long long origin;
 };
+
+__host__ __device__ __forceinline__ long long cell_index(const GridDesc &g, int i, int j, int k) {
+    return g.origin + (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+}
 
 // Lattice layouts of the in-place (A-A) scheme.  Post-collision population q of cell x:
 //   LAY_P (after an even-type sweep, and initially): slot INV[q] of x itself;
 //   LAY_Q (after an odd-type sweep): slot q of x+e_q, or slot INV[q] of x when x+e_q is
 //   solid (the half-way bounce-back is folded into the store).
-// An odd-type sweep turns P into Q, an even-type sweep Q into P.
-enum Layout : int { LAY_P = 0, LAY_Q = 1 };
+// An odd-type sweep turns P into Q, an even-type sweep Q into P.  Solid cells are never
+// swept; in LAY_P the slot q of a wall cell w holds the reflected f*_q(w - e_q) of its
+// fluid neighbour, so that the odd sweep reads bounced populations like streamed ones.
+//   LAY_AB (two arrays, pull sweep): slot q of x in the current array; the other array
+//   receives the next step.  In LAY_AB the slot INV[q] of a wall cell w holds the reflected
+//   f*_q(w - e_q) of its fluid neighbour, read by the next pull like a streamed population.
+enum Layout : int { LAY_P = 0, LAY_Q = 1, LAY_AB = 2 };
 
 struct StepArgs {
     void *pop;
+    void *pop_out;                       // LAY_AB: array receiving the step (else unused)
     const uint8_t *mask;
     const void *sig_x, *sig_y, *sig_z;   // per-axis sponge tables (compute type)
     GridDesc g;
@@ -71,12 +86,21 @@ struct StepArgs {
     int tiles_x, tiles_y;                // block tiling of the interior
     int nblocks;
     int xcd_remap;                       // map consecutive tiles to one XCD
+    int dpp;                             // every lane active: odd sweep may shift across lanes
+    void *fb[2];                         // x-face buffers (x-, x+) or null, see below
 };
+
+// x-face buffers.  The cells I = (1, j, k) and (nx, j, k) feed the NSCBC ghosts of the x
+// faces, but in the x-fastest layout their populations sit one per cache line.  The sweep
+// therefore also writes the post-collision populations of those cells, in storage form,
+// to fb[face][(j-1) + ny*(k-1)][q]; the source kernel adds to them like to the lattice, and
+// the ghost kernel reads them contiguously.
 
 struct GhostArgs {
     void *pop;
     const uint8_t *mask;
     GridDesc g;
+    const void *fb[2];                    // x-face buffers (see StepArgs)
     int nscbc_faces;
     const void *s_old;                    // tracked state of solid cells on open faces
     const int *face_map[6];               // per NSCBC face: plane index -> state slot or -1
@@ -88,6 +112,7 @@ struct SolidArgs {
     void *pop;
     const uint8_t *mask;
     GridDesc g;
+    int lay;
     const long long *cells;               // boundary solid cells
     int n;
     const void *s_old;
@@ -100,9 +125,11 @@ struct SourceArgs {
     void *pop;
     const uint8_t *mask;
     GridDesc g;
+    void *fb[2];                          // x-face buffers (see StepArgs)
     const long long *cells;               // fluid cells of this step's stencil
     const double *adds;                   // A*ww per cell (05:660)
     int n;
+    double alpha;                         // wall reflection (for the wall copies)
     float scale, inv_scale;
 };
 

@@ -145,8 +145,10 @@ struct nlos_engine {
     hipStream_t stream = nullptr;
     int device = 0;
 
-    DevBuf pop, mask, sig[3], flag, field, hist, probe_cells, s_a, s_b, solid_cells, src_cells,
-        src_adds, face_map[6], gh, vox_tri, vox_bins, vox_start, rho_tmp;
+    DevBuf pop, pop2, mask, sig[3], flag, field, hist, probe_cells, s_a, s_b, solid_cells, src_cells,
+        src_adds, face_map[6], gh, vox_tri, vox_bins, vox_start, rho_tmp, fb[2];
+    bool fb_valid = false;           // x-face buffers agree with the lattice
+    bool walls_valid = false;        // LAY_P wall copies agree with the lattice
     bool field_valid = false;
     bool field_alloc = false;
 
@@ -157,13 +159,16 @@ struct nlos_engine {
     long long hist_cap = 0;         // steps
     long long step = 0;
     int lay = LAY_P;
+    int pattern = NLOS_PATTERN_INPLACE;   // resolved stream pattern
+    int cur = 0;                          // two-array pattern: which array holds f*
 
     nlos_source src{};
     std::vector<double> samples;
     bool has_source = false;
 
     dim3 block{64, 4, 1};
-    int xcd_remap = 1;
+    int xcd_remap = 0;
+    int use_dpp = 1;
 
     bool timing = false;
     std::vector<hipEvent_t> ev;     // start/stop pairs
@@ -176,8 +181,10 @@ struct nlos_engine {
     std::vector<double> sched_adds;
 
     long long n_interior() const { return (long long)g.nx * g.ny * g.nz; }
+    void *curp() const { return cur == 0 ? pop.p : pop2.p; }
+    void *nxtp() const { return cur == 0 ? pop2.p : pop.p; }
     long long cell(int i, int j, int k) const {
-        return (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+        return cell_index(g, i, j, k);
     }
     bool solid_interior(int gi, int gj, int gk) const {
         return hmask[(size_t)gi + (size_t)g.nx * ((size_t)gj + (size_t)g.ny * gk)] != 0;
@@ -236,6 +243,8 @@ int upload_sponge(nlos_engine *e) {
 // Probe bits, NEAR bits, tracked solid cells on open faces; state reset to rest.
 int finalize_mask(nlos_engine *e) {
     GridDesc &g = e->g;
+    e->fb_valid = false;
+    e->walls_valid = false;
     HIP_TRY(launch_mask_flags(e->mask.as<uint8_t>(), g, e->stream));
     for (long long c : e->probe_ucells) {
         uint8_t b = 0;
@@ -436,24 +445,43 @@ int upload_schedule(nlos_engine *e) {
 }
 
 // ---- launch helpers -------------------------------------------------------------------
+int sync_face_buffers(nlos_engine *e);
+
 int launch_source(nlos_engine *e, long long off, long long n) {
     if (n <= 0) return NLOS_OK;
+    if (int rc = sync_face_buffers(e)) return rc;
     SourceArgs a{};
-    a.pop = e->pop.p;
+    a.pop = e->curp();
+    a.fb[0] = e->fb[0].p;
+    a.fb[1] = e->fb[1].p;
     a.mask = e->mask.as<uint8_t>();
     a.g = e->g;
     a.cells = e->src_cells.as<long long>() + off;
     a.adds = e->src_adds.as<double>() + off;
     a.n = (int)n;
+    a.alpha = e->cfg.wall_reflect;
     a.scale = e->scale;
     a.inv_scale = e->inv_scale;
     HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::source(a, e->lay, e->stream); }));
     return NLOS_OK;
 }
 
+int sync_face_buffers(nlos_engine *e) {
+    if (e->fb_valid || (!e->fb[0].p && !e->fb[1].p)) { e->fb_valid = true; return NLOS_OK; }
+    HIP_TRY(by_mode(e->mode, [&](auto L) {
+        return decltype(L)::fb_gather(e->curp(), e->mask.as<uint8_t>(), e->g, e->lay, e->fb[0].p, e->fb[1].p,
+                                      e->scale, e->inv_scale, e->stream);
+    }));
+    e->fb_valid = true;
+    return NLOS_OK;
+}
+
 int launch_halos(nlos_engine *e) {
+    if (int rc = sync_face_buffers(e)) return rc;
     GhostArgs a{};
-    a.pop = e->pop.p;
+    a.pop = e->curp();
+    a.fb[0] = e->fb[0].p;
+    a.fb[1] = e->fb[1].p;
     a.mask = e->mask.as<uint8_t>();
     a.g = e->g;
     a.nscbc_faces = e->cfg.nscbc_faces;
@@ -466,9 +494,10 @@ int launch_halos(nlos_engine *e) {
     HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::ghost(a, e->lay, e->stream); }));
     if (e->n_solid > 0) {
         SolidArgs s{};
-        s.pop = e->pop.p;
+        s.pop = e->curp();
         s.mask = e->mask.as<uint8_t>();
         s.g = e->g;
+        s.lay = e->lay;
         s.cells = e->solid_cells.as<long long>();
         s.n = e->n_solid;
         s.s_old = e->s_a.p;
@@ -491,7 +520,8 @@ int launch_sweep(nlos_engine *e, bool capture) {
         HIP_TRY(launch_fill_f32(e->field.as<float>(), e->n_interior(), 0.0f, e->stream));
     }
     StepArgs a{};
-    a.pop = e->pop.p;
+    a.pop = e->curp();
+    a.pop_out = e->nxtp();
     a.mask = e->mask.as<uint8_t>();
     a.sig_x = e->sig[0].p;
     a.sig_y = e->sig[1].p;
@@ -510,7 +540,18 @@ int launch_sweep(nlos_engine *e, bool capture) {
     a.tiles_y = (e->g.ny + (int)e->block.y - 1) / (int)e->block.y;
     a.nblocks = a.tiles_x * a.tiles_y * e->g.nz;
     a.xcd_remap = e->xcd_remap;
+    // lane shifts need full waves: rows of exactly 64-cell tiles, no partial block rows
+    a.dpp = e->use_dpp && e->block.x == 64 && e->g.nx % 64 == 0 && e->g.ny % (int)e->block.y == 0;
+    a.fb[0] = e->fb[0].p;
+    a.fb[1] = e->fb[1].p;
+    const bool two = e->pattern == NLOS_PATTERN_TWO_ARRAY;
     const bool odd = e->lay == LAY_P;
+    if ((two || odd) && !e->walls_valid)
+        HIP_TRY(by_mode(e->mode, [&](auto L) {
+            return decltype(L)::prime_walls(e->curp(), e->mask.as<uint8_t>(), e->g, e->lay, e->cfg.wall_reflect,
+                                            e->stream);
+        }));
+    e->walls_valid = true;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (e->timing) {
         if (e->ev_used + 2 > e->ev.size()) {
@@ -525,9 +566,14 @@ int launch_sweep(nlos_engine *e, bool capture) {
         e->ev_used += 2;
         HIP_TRY(hipEventRecord(t0, e->stream));
     }
-    HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::step(a, odd, capture, e->block, e->stream); }));
+    if (two) {
+        HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::step_ab(a, capture, e->block, e->stream); }));
+    } else {
+        HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::step(a, odd, capture, e->block, e->stream); }));
+    }
     if (e->timing) HIP_TRY(hipEventRecord(t1, e->stream));
-    e->lay = odd ? LAY_Q : LAY_P;
+    if (two) e->cur ^= 1;
+    else e->lay = odd ? LAY_Q : LAY_P;
     if (capture) e->field_valid = true;
     ++e->step;
     return NLOS_OK;
@@ -618,9 +664,11 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     e->cfg = *cfg;
     e->mode = cfg->precision;
     e->g.nx = NX; e->g.ny = NY; e->g.nz = NZ;
-    e->g.px = NX + 2; e->g.py = NY + 2; e->g.pz = NZ + 2;
+    e->g.px = ((NX + 2 + 63) / 64) * 64;
+    e->g.py = NY + 2; e->g.pz = NZ + 2;
     e->g.plane = (long long)e->g.px * e->g.py;
-    e->g.ncell = e->g.plane * e->g.pz;
+    e->g.origin = 63;
+    e->g.ncell = ((e->g.origin + e->g.plane * e->g.pz + 63) / 64) * 64;
     e->dt = (CS * cfg->dx) / cfg->c_phys;     // 05:200-203
     e->omega = 1.0 / cfg->tau;                // 05:908
     e->s_bytes = e->mode == M_F64 ? 8 : (e->mode == M_F32 ? 4 : 2);
@@ -632,6 +680,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     }
     if (const char *b = std::getenv("NLOS_BLOCK")) parse_block(b, e->block);
     if (const char *x = std::getenv("NLOS_XCD_REMAP")) e->xcd_remap = std::atoi(x) != 0;
+    if (const char *x = std::getenv("NLOS_DPP")) e->use_dpp = std::atoi(x) != 0;
 
     auto bail = [&](int rc) { nlos_destroy(e); return rc; };
     hipError_t he = hipGetDevice(&e->device);
@@ -639,15 +688,35 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)));
     const size_t popb = (size_t)e->g.ncell * Q * e->s_bytes;
+    e->pattern = cfg->stream_pattern;
+    if (e->pattern == NLOS_PATTERN_AUTO) {
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = 0;
+        // two arrays when both fit with room for the mask and work buffers
+        const size_t need = 2 * popb + (size_t)e->g.ncell + ((size_t)4 << 30);
+        e->pattern = freeb >= need ? NLOS_PATTERN_TWO_ARRAY : NLOS_PATTERN_INPLACE;
+    }
+    if (e->pattern != NLOS_PATTERN_INPLACE && e->pattern != NLOS_PATTERN_TWO_ARRAY)
+        return bail(fail(NLOS_ERR_ARG, "bad stream_pattern %d", cfg->stream_pattern));
+    e->lay = e->pattern == NLOS_PATTERN_TWO_ARRAY ? LAY_AB : LAY_P;
     if (e->pop.reserve(popb) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate %.2f GB of populations", popb / 1e9));
+    if (e->pattern == NLOS_PATTERN_TWO_ARRAY && e->pop2.reserve(popb) != hipSuccess)
+        return bail(fail(NLOS_ERR_NOMEM, "cannot allocate a second %.2f GB population array", popb / 1e9));
     if (e->mask.reserve((size_t)e->g.ncell) != hipSuccess || e->flag.reserve(sizeof(int)) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
+    for (int f = 0; f < 2; ++f)
+        if ((cfg->nscbc_faces >> f) & 1)
+            if (e->fb[f].reserve((size_t)NY * NZ * Q * e->s_bytes) != hipSuccess)
+                return bail(fail(NLOS_ERR_NOMEM, "cannot allocate x-face buffers"));
     int rc;
     if ((rc = upload_sponge(e)) != NLOS_OK) return bail(rc);
     if (hipMemsetAsync(e->flag.p, 0, sizeof(int), e->stream) != hipSuccess)
         return bail(fail(NLOS_ERR_HIP, "memset flag"));
     if (by_mode(e->mode, [&](auto L) { return decltype(L)::fill_rest(e->pop.p, e->g, e->stream); }) != hipSuccess)
+        return bail(fail(NLOS_ERR_HIP, "fill_rest launch failed"));
+    if (e->pop2.p &&
+        by_mode(e->mode, [&](auto L) { return decltype(L)::fill_rest(e->pop2.p, e->g, e->stream); }) != hipSuccess)
         return bail(fail(NLOS_ERR_HIP, "fill_rest launch failed"));
     // Empty geometry: mask from heights -1e30 everywhere (fluid, ground halo solid).
     VoxelArgs v{};
@@ -667,11 +736,11 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
 NLOS_API int nlos_destroy(nlos_engine *e) {
     if (!e) return NLOS_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    DevBuf *bufs[] = {&e->pop, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
+    DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                       &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
                       &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
                       &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
-                      &e->vox_bins, &e->vox_start, &e->rho_tmp};
+                      &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1]};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -686,12 +755,13 @@ NLOS_API int nlos_get_info(const nlos_engine *e, nlos_info *info) {
     info->nx = e->g.nx; info->ny = e->g.ny; info->nz = e->g.nz;
     info->precision = e->mode;
     info->dt = e->dt;
+    info->stream_pattern = e->pattern;
     info->step = e->step;
     info->n_probes = (int64_t)e->probes.size();
     info->n_solid_boundary = e->n_solid;
     info->bytes_per_update = 2.0 * Q * (double)e->s_bytes + 1.0;
     size_t tot = 0;
-    const DevBuf *bufs[] = {&e->pop, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
+    const DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                             &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
                             &e->src_adds, &e->gh};
     for (const DevBuf *b : bufs) tot += b->bytes;
@@ -797,10 +867,12 @@ NLOS_API int nlos_set_density(nlos_engine *e, const double *rho) {
     HIP_TRY(e->rho_tmp.reserve(n * 8));
     HIP_TRY(hipMemcpy(e->rho_tmp.p, rho, n * 8, hipMemcpyHostToDevice));
     HIP_TRY(by_mode(e->mode, [&](auto L) {
-        return decltype(L)::set_density(e->pop.p, e->mask.as<uint8_t>(), e->g, e->rho_tmp.as<double>(), e->scale, e->stream);
+        return decltype(L)::set_density(e->curp(), e->mask.as<uint8_t>(), e->g, e->rho_tmp.as<double>(), e->scale, e->stream);
     }));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->rho_tmp.release();
+    e->fb_valid = false;
+    e->walls_valid = false;
     return NLOS_OK;
 }
 
@@ -1014,7 +1086,7 @@ NLOS_API int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *need
     char *p = static_cast<char *>(buf);
     std::memcpy(p, &h, sizeof h);
     p += sizeof h;
-    HIP_TRY(hipMemcpy(p, e->pop.p, h.pop_bytes, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(p, e->curp(), h.pop_bytes, hipMemcpyDeviceToHost));
     p += h.pop_bytes;
     if (h.state_bytes) HIP_TRY(hipMemcpy(p, e->s_a.p, h.state_bytes, hipMemcpyDeviceToHost));
     p += h.state_bytes;
@@ -1028,13 +1100,15 @@ NLOS_API int nlos_restore(nlos_engine *e, const void *buf, size_t len) {
     CkptHeader h;
     std::memcpy(&h, buf, sizeof h);
     if (std::memcmp(h.magic, "NLOSCKP1", 8) != 0) return fail(NLOS_ERR_ARG, "not a checkpoint");
+    if ((h.lay == LAY_AB) != (e->pattern == NLOS_PATTERN_TWO_ARRAY))
+        return fail(NLOS_ERR_ARG, "checkpoint was taken with the other stream pattern");
     if (h.mode != e->mode || h.nx != e->g.nx || h.ny != e->g.ny || h.nz != e->g.nz || h.ncell != e->g.ncell ||
         h.n_solid != e->n_solid || h.n_ucells != (int64_t)e->probe_ucells.size())
         return fail(NLOS_ERR_ARG, "checkpoint does not match this engine's grid, geometry or probes");
     if (len < sizeof h + h.pop_bytes + h.state_bytes + h.hist_bytes) return fail(NLOS_ERR_ARG, "checkpoint truncated");
     const char *p = static_cast<const char *>(buf) + sizeof h;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    HIP_TRY(hipMemcpy(e->pop.p, p, h.pop_bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->curp(), p, h.pop_bytes, hipMemcpyHostToDevice));
     p += h.pop_bytes;
     if (h.state_bytes) HIP_TRY(hipMemcpy(e->s_a.p, p, h.state_bytes, hipMemcpyHostToDevice));
     p += h.state_bytes;
@@ -1044,6 +1118,8 @@ NLOS_API int nlos_restore(nlos_engine *e, const void *buf, size_t len) {
     e->step = h.step;
     e->lay = h.lay;
     e->field_valid = false;
+    e->fb_valid = false;
+    e->walls_valid = false;
     HIP_TRY(hipMemsetAsync(e->flag.p, 0, sizeof(int), e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return NLOS_OK;
@@ -1081,7 +1157,7 @@ NLOS_API int nlos_debug_read_f(nlos_engine *e, double *out) {
     DevBuf tmp;
     HIP_TRY(tmp.reserve((size_t)e->n_interior() * Q * 8));
     HIP_TRY(by_mode(e->mode, [&](auto L) {
-        return decltype(L)::gather_f(e->pop.p, e->mask.as<uint8_t>(), e->g, e->lay, e->inv_scale, tmp.as<double>(), e->stream);
+        return decltype(L)::gather_f(e->curp(), e->mask.as<uint8_t>(), e->g, e->lay, e->inv_scale, tmp.as<double>(), e->stream);
     }));
     HIP_TRY(hipStreamSynchronize(e->stream));
     HIP_TRY(hipMemcpy(out, tmp.p, (size_t)e->n_interior() * Q * 8, hipMemcpyDeviceToHost));
@@ -1097,14 +1173,44 @@ NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, in
         return fail(NLOS_ERR_ARG, "poke outside grid");
     HIP_TRY(hipStreamSynchronize(e->stream));
     const long long idx = (long long)q * e->g.ncell + e->cell(i + 1, j + 1, k + 1);
+    e->fb_valid = false;
+    e->walls_valid = false;
     if (e->mode == M_F64) {
-        HIP_TRY(hipMemcpy(e->pop.as<double>() + idx, &f, 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(static_cast<double *>(e->curp()) + idx, &f, 8, hipMemcpyHostToDevice));
     } else if (e->mode == M_F32) {
         const float v = (float)(f - W64[q]);
-        HIP_TRY(hipMemcpy(e->pop.as<float>() + idx, &v, 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(static_cast<float *>(e->curp()) + idx, &v, 4, hipMemcpyHostToDevice));
     } else {
         const _Float16 v = (_Float16)((float)(f - W64[q]) * e->scale);
-        HIP_TRY(hipMemcpy(static_cast<_Float16 *>(e->pop.p) + idx, &v, 2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(static_cast<_Float16 *>(e->curp()) + idx, &v, 2, hipMemcpyHostToDevice));
     }
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) {
+    if (bytes < 16 || reps < 1 || !avg_ms) return fail(NLOS_ERR_ARG, "bad copy request");
+    const long long n16 = (long long)(bytes / 16);
+    DevBuf a, b;
+    HIP_TRY(a.reserve((size_t)n16 * 16));
+    HIP_TRY(b.reserve((size_t)n16 * 16));
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipMemsetAsync(a.p, 0, (size_t)n16 * 16, s));
+    HIP_TRY(launch_copy16(a.p, b.p, n16, s));
+    HIP_TRY(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) HIP_TRY(launch_copy16(a.p, b.p, n16, s));
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    a.release();
+    b.release();
     return NLOS_OK;
 }

@@ -136,19 +136,35 @@ __device__ __forceinline__ float geq32(int q, float dr, float r, float ux, float
 }
 
 // BGK collision in deviation form with the sponge as a factor (1 - sigma) on g
-// ((1-s)f + s*w - w = (1-s)g).  Returns drho.
-__device__ __forceinline__ float collide32(float *g, float omega, float keep) {
-    float dr, jx, jy, jz;
-    moments32(g, dr, jx, jy, jz);
+// ((1-s)f + s*w - w = (1-s)g).  Works on the stored values G = g*S (S = 1 for FP32, 2^k
+// for FP16): the moments are scaled back once, the equilibrium is formed in stored units,
+// and a direction and its inverse share the even part of the equilibrium
+// (cu_inv = -cu).  Returns drho.
+__device__ __forceinline__ float collide32(float *G, float omega, float keep, float S, float invS) {
+    float drs, jx, jy, jz;
+    moments32(G, drs, jx, jy, jz);
+    float dr = drs * invS;
     float r = 1.0f + dr;
-    if (r <= 0.0f) { r = 1e-12f; dr = r - 1.0f; }
-    const float ir = 1.0f / r;
-    const float ux = jx * ir, uy = jy * ir, uz = jz * ir;
+    if (r <= 0.0f) { r = 1e-12f; dr = r - 1.0f; drs = dr * S; }
+    const float k = invS / r;
+    const float ux = jx * k, uy = jy * k, uz = jz * k;
     const float u2c = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float rS = r * S;
+    const float om1 = 1.0f - omega;
+    {
+        const float ge = wq<float>(0) * (drs - rS * u2c);
+        G[0] = keep * (om1 * G[0] + omega * ge);
+    }
+    constexpr int PAIR[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};   // one of each inverse pair
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const float ge = geq32(q, dr, r, ux, uy, uz, u2c);
-        g[q] = keep * (g[q] + omega * (ge - g[q]));
+    for (int n = 0; n < 9; ++n) {
+        const int q = PAIR[n], p = INV[q];
+        const float cu = cu_of<float>(q, ux, uy, uz);
+        const float wr = wq<float>(q) * rS;
+        const float even = wq<float>(q) * drs + wr * (4.5f * cu * cu - u2c);
+        const float odd = wr * (3.0f * cu);
+        G[q] = keep * (om1 * G[q] + omega * (even + odd));
+        G[p] = keep * (om1 * G[p] + omega * (even - odd));
     }
     return dr;
 }
@@ -181,55 +197,143 @@ __device__ __forceinline__ int probe_slot(const long long *cells, int n, long lo
     return -1;
 }
 
+// Per-lane load/store through a wave-uniform base pointer and a 32-bit lane offset, so
+// that the address is an SGPR base plus a VGPR offset (no per-lane 64-bit arithmetic).
+// Built with -DNLOS_NONTEMPORAL the sweep's streaming accesses carry the non-temporal hint.
+template <class S>
+__device__ __forceinline__ S ld_stream(const S *p) {
+#ifdef NLOS_NONTEMPORAL
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
+template <class S>
+__device__ __forceinline__ void st_stream(S *p, S v) {
+#ifdef NLOS_NONTEMPORAL
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+template <int MODE>
+__device__ __forceinline__ typename Tr<MODE>::C ldu(const typename Tr<MODE>::S *base, unsigned t, float inv_scale) {
+    if constexpr (MODE == M_F16) return static_cast<float>(ld_stream(base + t)) * inv_scale;
+    else return ld_stream(base + t);
+}
+
+template <int MODE>
+__device__ __forceinline__ void stu(typename Tr<MODE>::S *base, unsigned t, typename Tr<MODE>::C v, float scale) {
+    using S = typename Tr<MODE>::S;
+    if constexpr (MODE == M_F16) st_stream(base + t, static_cast<S>(v * scale));
+    else st_stream(base + t, v);
+}
+
+// Whole-wave shift by one lane: with right = true lane L receives lane L-1's value and lane
+// 0 receives `edge`; with right = false lane L receives lane L+1's value and lane 63 `edge`.
+// Needs every lane of the wave active.
+__device__ __forceinline__ int dpp_shift(int v, int edge, bool right) {
+    return right ? __builtin_amdgcn_update_dpp(edge, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false)
+                 : __builtin_amdgcn_update_dpp(edge, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+}
+
+template <class S>
+__device__ __forceinline__ S wave_shift(S v, S edge, bool right) {
+    if constexpr (sizeof(S) == 2) {
+        const int r = dpp_shift((int)__builtin_bit_cast(unsigned short, v),
+                                (int)__builtin_bit_cast(unsigned short, edge), right);
+        return __builtin_bit_cast(S, (unsigned short)r);
+    } else if constexpr (sizeof(S) == 4) {
+        return __builtin_bit_cast(S, dpp_shift(__builtin_bit_cast(int, v), __builtin_bit_cast(int, edge), right));
+    } else {
+        const long long bv = __builtin_bit_cast(long long, v), be = __builtin_bit_cast(long long, edge);
+        const int lo = dpp_shift((int)bv, (int)be, right);
+        const int hi = dpp_shift((int)(bv >> 32), (int)(be >> 32), right);
+        return __builtin_bit_cast(S, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    }
+}
+
 template <int MODE, bool ODD, bool CAPTURE>
 __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     using C = typename Tr<MODE>::C;
+    using S = typename Tr<MODE>::S;
+    // Values stay in stored units through the sweep (scaled FP16 is converted, not rescaled).
+    constexpr float RAW = 1.0f;
     int b = blockIdx.x;
     if (a.xcd_remap) b = remap_block(b, gridDim.x);
     const int bi = b % a.tiles_x;
     const int rest_b = b / a.tiles_x;
     const int bj = rest_b % a.tiles_y;
     const int k = rest_b / a.tiles_y + 1;
-    const int i = bi * (int)blockDim.x + (int)threadIdx.x + 1;
-    const int j = bj * (int)blockDim.y + (int)threadIdx.y + 1;
+    const int i0 = bi * (int)blockDim.x + 1;
+    const int j0 = bj * (int)blockDim.y + 1;
+    const int i = i0 + (int)threadIdx.x;
+    const int j = j0 + (int)threadIdx.y;
     if (i > a.g.nx || j > a.g.ny) return;
 
     const long long N = a.g.ncell;
-    const long long c = (long long)i + (long long)a.g.px * ((long long)j + (long long)a.g.py * k);
-    const uint8_t m = a.mask[c];
-    if (m & MK_SOLID) return;
-
-    // Solid-neighbour bits: bit q set when c + e_q is solid.
-    uint32_t sb = 0;
-    const bool need_nb = (m & MK_NEAR) && (ODD || a.alpha != 1.0);
-    if (need_nb) {
-#pragma unroll
-        for (int q = 1; q < Q; ++q)
-            if (a.mask[c + nb_off(a.g, q)] & MK_SOLID) sb |= 1u << q;
-    }
-
-    C f[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const bool from_wall = (sb >> INV[q]) & 1u;     // c - e_q is solid
-        if constexpr (ODD) {
-            // LAY_P: f*_q(c - e_q) sits in slot INV[q] of c - e_q; a bounced f*_{INV q}(c)
-            // sits in slot q of c.
-            const long long idx = from_wall ? (long long)q * N + c
-                                            : (long long)INV[q] * N + c - nb_off(a.g, q);
-            f[q] = ld<MODE>(a.pop, idx, a.inv_scale);
-        } else {
-            // LAY_Q: both the streamed and the bounced population sit in slot q of c.
-            f[q] = ld<MODE>(a.pop, (long long)q * N + c, a.inv_scale);
-        }
-        if (from_wall && a.alpha != 1.0) f[q] = reflect<MODE>(q, f[q], a.alpha);
-    }
-
+    const long long cb = cell_index(a.g, i0, j0, k);                  // block corner, uniform
+    const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;  // lane offset
+    const long long c = cb + t;
+    S *P = static_cast<S *>(a.pop);
     const C *sx = static_cast<const C *>(a.sig_x);
     const C *sy = static_cast<const C *>(a.sig_y);
     const C *sz = static_cast<const C *>(a.sig_z);
+
+    // Every load is issued before the mask is looked at, so that the mask byte adds no
+    // memory round trip in front of the population loads; all addresses are inside the
+    // lattice for any interior cell (solid cells included).
+    //   ODD (LAY_P): f_in_q(c) = f*_q(c - e_q) from slot INV[q] of c - e_q.  When c - e_q is
+    //     solid, that slot of the solid cell holds the bounced f*_{INV q}(c): the previous
+    //     even sweep (or k_prime_walls) put it there.
+    //   EVEN (LAY_Q): streamed and bounced populations both sit in slot q of c.
+    const uint8_t m = (a.mask + cb)[t];
+    C f[Q];
+    if (ODD && a.dpp) {
+        // Odd sweep with every lane active: a wave is one aligned row of 64 cells, so a
+        // source row shifted by one cell in x is read as its aligned row plus one edge
+        // element and moved across lanes, instead of as a load straddling two lines.
+        // signed: the left edge element sits one before the row start
+        const long long row = (long long)a.g.px * (long long)threadIdx.y;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const long long rowoff = (long long)EY[q] * a.g.px + (long long)EZ[q] * a.g.plane;
+            const S *base = P + ((long long)INV[q] * N + cb - rowoff);
+            if (EX[q] == 0) {
+                f[q] = ldu<MODE>(base, t, RAW);
+            } else {
+                const S v = ld_stream(base + t);
+                const S edge = EX[q] > 0 ? base[row - 1] : base[row + 64];
+                f[q] = (C)wave_shift<S>(v, edge, EX[q] > 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const S *src = ODD ? P + ((long long)INV[q] * N + cb - nb_off(a.g, q)) : P + ((long long)q * N + cb);
+            f[q] = ldu<MODE>(src, t, RAW);
+        }
+    }
     // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
     const C sig = fmax(sx[i], fmax(sy[j], sz[k]));
+    if (m & MK_SOLID) return;
+    const bool near = m & MK_NEAR;
+
+    uint32_t sb = 0;   // bit q: c + e_q is solid (only gathered for cells next to a wall)
+    if (near) {
+#pragma unroll
+        for (int q = 1; q < Q; ++q)
+            if (a.mask[c + nb_off(a.g, q)] & MK_SOLID) sb |= 1u << q;
+        // EVEN: a population that came off a wall is reflected here (ODD: the reflected
+        // value was stored in the wall's slot).
+        if (!ODD && a.alpha != 1.0) {
+#pragma unroll
+            for (int q = 1; q < Q; ++q)
+                if ((sb >> INV[q]) & 1u) f[q] = reflect<MODE>(q, f[q], a.alpha);
+        }
+    }
 
     double dval;   // rho - 1
     bool finite;
@@ -238,7 +342,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
         dval = r - 1.0;
         finite = isfinite(r);
     } else {
-        const float dr = collide32(f, (float)a.omega, 1.0f - sig);
+        const float dr = collide32(f, (float)a.omega, 1.0f - sig, a.scale, a.inv_scale);
         dval = dr;
         finite = isfinite(dr);
     }
@@ -251,19 +355,163 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
         a.field[(long long)(i - 1) + (long long)a.g.nx * ((long long)(j - 1) + (long long)a.g.ny * (k - 1))] =
             (float)dval;
     }
+    if (i == 1 || i == a.g.nx) {
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            if ((face == 0 ? i != 1 : i != a.g.nx) || !a.fb[face]) continue;
+            S *fb = static_cast<S *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) stu<MODE>(fb, q, f[q], RAW);
+        }
+    }
 
+    if constexpr (ODD) {
+        // -> LAY_Q: slot q of c + e_q; when c + e_q is solid, slot INV[q] of c (half-way
+        // bounce-back folded into the store; the copy into the wall's slot is never read).
+#pragma unroll
+        for (int q = 0; q < Q; ++q) stu<MODE>(P + ((long long)q * N + cb + nb_off(a.g, q)), t, f[q], RAW);
+        if (sb) {
+#pragma unroll
+            for (int q = 1; q < Q; ++q)
+                if ((sb >> q) & 1u) st<MODE>(a.pop, (long long)INV[q] * N + c, f[q], RAW);
+        }
+    } else {
+        // -> LAY_P: slot INV[q] of c; a population headed into a wall is also stored,
+        // reflected, in slot q of the wall cell c + e_q, where the next odd sweep reads it.
+#pragma unroll
+        for (int q = 0; q < Q; ++q) stu<MODE>(P + ((long long)INV[q] * N + cb), t, f[q], RAW);
+        if (sb) {
+#pragma unroll
+            for (int q = 1; q < Q; ++q)
+                if ((sb >> q) & 1u)
+                    st<MODE>(a.pop, (long long)q * N + c + nb_off(a.g, q),
+                             a.alpha != 1.0 ? reflect<MODE>(q, f[q], a.alpha) : f[q], RAW);
+        }
+    }
+}
+
+// Byte-offset access through a wave-uniform base: the address is an SGPR base plus a
+// 32-bit VGPR offset (global_load ... saddr), with no per-lane 64-bit arithmetic.
+template <class S>
+__device__ __forceinline__ S ld_b(const S *base, unsigned tb) {
+    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + tb);
+}
+
+template <class S>
+__device__ __forceinline__ void st_b(S *base, unsigned tb, S v) {
+    *reinterpret_cast<S *>(reinterpret_cast<char *>(base) + tb) = v;
+}
+
+// Pull sweep of the two-array pattern (LAY_AB, 05:917-968): f_in_q(c) = cur[q][c - e_q]
+// (the wall copy when c - e_q is solid), collide, write nxt[q][c] (aligned, local); a cell
+// next to a wall also writes the reflected f*_q(c) into slot INV[q] of the wall cell
+// c + e_q of nxt.
+template <int MODE, bool CAPTURE>
+__global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
+    using C = typename Tr<MODE>::C;
+    using S = typename Tr<MODE>::S;
+    int b = blockIdx.x;
+    if (a.xcd_remap) b = remap_block(b, gridDim.x);
+    const int bi = b % a.tiles_x;
+    const int rest_b = b / a.tiles_x;
+    const int bj = rest_b % a.tiles_y;
+    const int k = rest_b / a.tiles_y + 1;
+    const int i0 = bi * (int)blockDim.x + 1;
+    const int j0 = bj * (int)blockDim.y + 1;
+    const int i = i0 + (int)threadIdx.x;
+    const int j = j0 + (int)threadIdx.y;
+    if (i > a.g.nx || j > a.g.ny) return;
+
+    const long long N = a.g.ncell;
+    const long long cb = cell_index(a.g, i0, j0, k);                  // block corner, uniform
+    const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;  // lane offset (cells)
+    const unsigned tb = t * (unsigned)sizeof(S);                      // lane offset (bytes)
+    const long long c = cb + t;
+    const S *cur = static_cast<const S *>(a.pop);
+    S *nxt = static_cast<S *>(a.pop_out);
+    const C *sx = static_cast<const C *>(a.sig_x);
+    const C *sy = static_cast<const C *>(a.sig_y);
+    const C *sz = static_cast<const C *>(a.sig_z);
+
+    // All loads first; the addresses are inside the lattice for any interior cell.
+    const uint8_t m = (a.mask + cb)[t];
+    C f[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        if constexpr (ODD) {
-            // -> LAY_Q: slot q of c + e_q, or slot INV[q] of c when c + e_q is solid.
-            const bool to_wall = (sb >> q) & 1u;
-            const long long idx = to_wall ? (long long)INV[q] * N + c
-                                          : (long long)q * N + c + nb_off(a.g, q);
-            st<MODE>(a.pop, idx, f[q], a.scale);
-        } else {
-            // -> LAY_P: slot INV[q] of c.
-            st<MODE>(a.pop, (long long)INV[q] * N + c, f[q], a.scale);
+        const S v = ld_b(cur + ((long long)q * N + cb - nb_off(a.g, q)), tb);
+        if constexpr (MODE == M_F16) f[q] = static_cast<float>(v);
+        else f[q] = v;
+    }
+    const C sig = fmax(sx[i], fmax(sy[j], sz[k]));
+    if (m & MK_SOLID) return;
+
+    double dval;   // rho - 1
+    bool finite;
+    if constexpr (MODE == M_F64) {
+        const double r = collide64(f, a.omega, sig);
+        dval = r - 1.0;
+        finite = isfinite(r);
+    } else {
+        const float dr = collide32(f, (float)a.omega, 1.0f - sig, a.scale, a.inv_scale);
+        dval = dr;
+        finite = isfinite(dr);
+    }
+    if (!finite) atomicOr(a.flag, 1);
+    if (m & MK_PROBE) {
+        const int u = probe_slot(a.probe_cells, a.n_probe_cells, c);
+        if (u >= 0) a.probe_row[u] = dval;
+    }
+    if constexpr (CAPTURE) {
+        a.field[(long long)(i - 1) + (long long)a.g.nx * ((long long)(j - 1) + (long long)a.g.ny * (k - 1))] =
+            (float)dval;
+    }
+    if (i == 1 || i == a.g.nx) {
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            if ((face == 0 ? i != 1 : i != a.g.nx) || !a.fb[face]) continue;
+            S *fb = static_cast<S *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fb[q] = static_cast<S>(f[q]);
         }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) st_b(nxt + ((long long)q * N + cb), tb, static_cast<S>(f[q]));
+    if (m & MK_NEAR) {
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            const long long w = c + nb_off(a.g, q);
+            if (a.mask[w] & MK_SOLID) {
+                C v = f[q];
+                if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
+                nxt[(long long)INV[q] * N + w] = static_cast<S>(v);
+            }
+        }
+    }
+}
+
+// Wall copies of LAY_P and LAY_AB (see Layout): for every fluid cell c next to a wall and
+// every q with c + e_q solid, the wall cell receives the reflected f*_q(c).  Needed after
+// the state was set from outside.
+template <int MODE, int LAY>
+__global__ __launch_bounds__(256) void k_prime_walls(void *pop, const uint8_t *mask, GridDesc g, double alpha) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    if (n >= total) return;
+    const int i = (int)(n % g.nx) + 1;
+    const int j = (int)((n / g.nx) % g.ny) + 1;
+    const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
+    const long long c = cell_index(g, i, j, k);
+    const uint8_t m = mask[c];
+    if ((m & MK_SOLID) || !(m & MK_NEAR)) return;
+    const long long N = g.ncell;
+    for (int q = 1; q < Q; ++q) {
+        const long long t = c + nb_off(g, q);
+        if (!(mask[t] & MK_SOLID)) continue;
+        // LAY_P: f*_q(c) in slot INV[q] of c, copy to slot q of the wall;
+        // LAY_AB: f*_q(c) in slot q of c, copy to slot INV[q] of the wall
+        auto v = ld<MODE>(pop, (long long)(LAY == LAY_AB ? q : INV[q]) * N + c, 1.0f);
+        if (alpha != 1.0) v = reflect<MODE>(q, v, alpha);
+        st<MODE>(pop, (long long)(LAY == LAY_AB ? INV[q] : q) * N + t, v, 1.0f);
     }
 }
 
@@ -274,7 +522,9 @@ __device__ __forceinline__ typename Tr<MODE>::C read_post(const void *pop, const
                                                           const GridDesc &g, long long c, int q,
                                                           float inv_scale) {
     const long long N = g.ncell;
-    if constexpr (LAY == LAY_P) {
+    if constexpr (LAY == LAY_AB) {
+        return ld<MODE>(pop, (long long)q * N + c, inv_scale);
+    } else if constexpr (LAY == LAY_P) {
         return ld<MODE>(pop, (long long)INV[q] * N + c, inv_scale);
     } else {
         const long long t = c + nb_off(g, q);
@@ -316,7 +566,7 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
         j = (int)(r % ny) + 1;
     }
     const long long N = g.ncell;
-    const long long h = (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    const long long h = cell_index(g, i, j, k);
     if (a.mask[h] & MK_SOLID) return;
 
     const int oi = i == 0 ? -1 : (i == nx + 1 ? 1 : 0);
@@ -336,8 +586,13 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
     C rI = 1, uI[3] = {0, 0, 0}, rB = 1, uB[3] = {0, 0, 0}, dI = 0, dB = 0, u2I = 0, u2B = 0;
     if (ns) {
         const int ii = i - oi, jj = j - oj, kk = k - ok;
-        const long long I = (long long)ii + (long long)g.px * ((long long)jj + (long long)g.py * kk);
-        if (a.mask[I] & MK_SOLID) {
+        const long long I = cell_index(g, ii, jj, kk);
+        if (axis == 0 && a.fb[face & 1] && !(a.mask[I] & MK_SOLID)) {
+            const typename Tr<MODE>::S *fb = static_cast<const typename Tr<MODE>::S *>(a.fb[face & 1]) +
+                                             ((long long)(jj - 1) + (long long)ny * (kk - 1)) * Q;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fI[q] = ldu<MODE>(fb, q, a.inv_scale);
+        } else if (a.mask[I] & MK_SOLID) {
             int pidx;
             if (axis == 0) pidx = (jj - 1) + ny * (kk - 1);
             else if (axis == 1) pidx = (ii - 1) + nx * (kk - 1);
@@ -346,6 +601,14 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
             const C *so = static_cast<const C *>(a.s_old);
 #pragma unroll
             for (int q = 0; q < Q; ++q) fI[q] = so[(long long)slot * Q + q];
+        } else if (LAY != LAY_Q || !(a.mask[I] & MK_NEAR)) {
+            // f*_q(I) in slot q of I (LAY_AB), slot INV[q] of I (LAY_P), or, with no solid
+            // neighbour, slot q of I + e_q (LAY_Q)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                fI[q] = ld<MODE>(a.pop, LAY == LAY_AB ? (long long)q * g.ncell + I
+                                        : LAY == LAY_P ? (long long)INV[q] * g.ncell + I
+                                                       : (long long)q * g.ncell + I + nb_off(g, q), a.inv_scale);
         } else {
 #pragma unroll
             for (int q = 0; q < Q; ++q) fI[q] = read_post<MODE, LAY>(a.pop, a.mask, g, I, q, a.inv_scale);
@@ -412,7 +675,9 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
         }
         const long long tc = h + nb_off(g, q);
         long long idx;
-        if constexpr (LAY == LAY_P) {
+        if constexpr (LAY == LAY_AB) {
+            idx = (long long)q * N + h;
+        } else if constexpr (LAY == LAY_P) {
             idx = (long long)INV[q] * N + h;
         } else {
             idx = (a.mask[tc] & MK_SOLID) ? (long long)INV[q] * N + h : (long long)q * N + tc;
@@ -423,8 +688,8 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
 
 // ---- solid cells on open faces (05:917-938 for a SOLID dst) -------------------------------
 // f_next[b][q] = solid(b - e_q) ? f[b][inv q] : f*_q(b - e_q).  With b solid, f*_q of a
-// fluid neighbour sits in slot INV[q] of that neighbour in either layout; the ghost kernel
-// puts a halo neighbour's value in the same place.
+// fluid neighbour sits in slot INV[q] of that neighbour in either A-A layout and in slot q
+// of it in LAY_AB; the ghost kernel puts a halo neighbour's value in the same place.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_solid(SolidArgs a) {
     using C = typename Tr<MODE>::C;
@@ -442,7 +707,7 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs a) {
             v = so[(long long)n * Q + INV[q]];
             if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
         } else {
-            v = ld<MODE>(a.pop, (long long)INV[q] * N + src, a.inv_scale);
+            v = ld<MODE>(a.pop, (long long)(a.lay == LAY_AB ? q : INV[q]) * N + src, a.inv_scale);
         }
         sn[(long long)n * Q + q] = v;
     }
@@ -457,19 +722,66 @@ __global__ __launch_bounds__(256) void k_source(SourceArgs a) {
     const long long c = a.cells[e];
     const long long N = a.g.ncell;
     long long idx;
-    if constexpr (LAY == LAY_P) {
+    if constexpr (LAY == LAY_AB) {
+        idx = (long long)q * N + c;
+    } else if constexpr (LAY == LAY_P) {
         idx = (long long)INV[q] * N + c;
     } else {
         const long long tc = c + nb_off(a.g, q);
         idx = (a.mask[tc] & MK_SOLID) ? (long long)INV[q] * N + c : (long long)q * N + tc;
     }
+    typename Tr<MODE>::C v;
     if constexpr (MODE == M_F64) {
         double *p = static_cast<double *>(a.pop);
         p[idx] += W64[q] * a.adds[e];
+        v = p[idx];
     } else {
-        const float v = ld<MODE>(a.pop, idx, a.inv_scale) + (float)(W64[q] * a.adds[e]);
+        v = ld<MODE>(a.pop, idx, a.inv_scale) + (float)(W64[q] * a.adds[e]);
         st<MODE>(a.pop, idx, v, a.scale);
     }
+    if constexpr (LAY != LAY_Q) {
+        // keep the wall copy of a population headed into a wall in step (see Layout)
+        const long long tc = c + nb_off(a.g, q);
+        if (q != 0 && (a.mask[tc] & MK_SOLID))
+            st<MODE>(a.pop, (long long)(LAY == LAY_AB ? INV[q] : q) * N + tc,
+                     a.alpha != 1.0 ? reflect<MODE>(q, v, a.alpha) : v, a.scale);
+    }
+    // the x-face buffer holds the same f*_q of boundary cells; apply the same add to it
+    const long long r = c - a.g.origin;
+    const int i = (int)(r % a.g.px);
+    if ((i == 1 && a.fb[0]) || (i == a.g.nx && a.fb[1])) {
+        const int j = (int)((r / a.g.px) % a.g.py);
+        const int k = (int)(r / a.g.plane);
+        const long long fidx = ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q + q;
+        for (int face = 0; face < 2; ++face) {
+            if ((face == 0 ? i != 1 : i != a.g.nx) || !a.fb[face]) continue;
+            if constexpr (MODE == M_F64) {
+                static_cast<double *>(a.fb[face])[fidx] += W64[q] * a.adds[e];
+            } else {
+                const float v = ld<MODE>(a.fb[face], fidx, a.inv_scale) + (float)(W64[q] * a.adds[e]);
+                st<MODE>(a.fb[face], fidx, v, a.scale);
+            }
+        }
+    }
+}
+
+// Fills the x-face buffers from the lattice (after the state was set from outside).
+template <int MODE, int LAY>
+__global__ __launch_bounds__(256) void k_fb_gather(const void *pop, const uint8_t *mask, GridDesc g,
+                                                  void *fb0, void *fb1, float scale, float inv_scale) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nface = (long long)g.ny * g.nz;
+    if (t >= 2 * nface) return;
+    const int face = t < nface ? 0 : 1;
+    void *fb = face == 0 ? fb0 : fb1;
+    if (!fb) return;
+    const long long jk = face == 0 ? t : t - nface;
+    const int j = (int)(jk % g.ny) + 1, k = (int)(jk / g.ny) + 1;
+    const long long I = cell_index(g, face == 0 ? 1 : g.nx, j, k);
+    if (mask[I] & MK_SOLID) return;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        st<MODE>(fb, jk * Q + q, read_post<MODE, LAY>(pop, mask, g, I, q, inv_scale), scale);
 }
 
 // ---- initialisation and gathers ------------------------------------------------------------
@@ -490,7 +802,7 @@ __global__ __launch_bounds__(256) void k_set_density(void *pop, const uint8_t *m
     const int i = (int)(n % g.nx) + 1;
     const int j = (int)((n / g.nx) % g.ny) + 1;
     const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
-    const long long c = (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    const long long c = cell_index(g, i, j, k);
     if (mask[c] & MK_SOLID) return;
     const double r = rho[n];
 #pragma unroll
@@ -519,7 +831,7 @@ __global__ __launch_bounds__(256) void k_gather_f(const void *pop, const uint8_t
     const int i = (int)(n % g.nx) + 1;
     const int j = (int)((n / g.nx) % g.ny) + 1;
     const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
-    const long long c = (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    const long long c = cell_index(g, i, j, k);
     const bool solid = mask[c] & MK_SOLID;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -549,11 +861,20 @@ hipError_t Launch<MODE>::step(const StepArgs &a, bool odd, bool capture, dim3 bl
 }
 
 template <int MODE>
+hipError_t Launch<MODE>::step_ab(const StepArgs &a, bool capture, dim3 block, hipStream_t s) {
+    const dim3 grid(a.nblocks);
+    if (capture) k_step_ab<MODE, true><<<grid, block, 0, s>>>(a);
+    else k_step_ab<MODE, false><<<grid, block, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int MODE>
 hipError_t Launch<MODE>::ghost(const GhostArgs &a, int lay, hipStream_t s) {
     if (a.n_halo <= 0) return hipSuccess;
     const unsigned nb = blocks_for(a.n_halo, 256);
     if (lay == LAY_P) k_ghost<MODE, LAY_P><<<nb, 256, 0, s>>>(a);
-    else k_ghost<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
+    else if (lay == LAY_Q) k_ghost<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
+    else k_ghost<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -569,7 +890,8 @@ hipError_t Launch<MODE>::source(const SourceArgs &a, int lay, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
     const unsigned nb = blocks_for((long long)a.n * Q, 256);
     if (lay == LAY_P) k_source<MODE, LAY_P><<<nb, 256, 0, s>>>(a);
-    else k_source<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
+    else if (lay == LAY_Q) k_source<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
+    else k_source<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -595,11 +917,31 @@ hipError_t Launch<MODE>::fill_state_rest(void *state, int n, hipStream_t s) {
 }
 
 template <int MODE>
+hipError_t Launch<MODE>::prime_walls(void *pop, const uint8_t *mask, const GridDesc &g, int lay, double alpha,
+                                     hipStream_t s) {
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    if (lay == LAY_AB) k_prime_walls<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, alpha);
+    else k_prime_walls<MODE, LAY_P><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, alpha);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::fb_gather(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
+                                   void *fb0, void *fb1, float scale, float inv_scale, hipStream_t s) {
+    const long long n = 2LL * g.ny * g.nz;
+    if (lay == LAY_P) k_fb_gather<MODE, LAY_P><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+    else if (lay == LAY_Q) k_fb_gather<MODE, LAY_Q><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+    else k_fb_gather<MODE, LAY_AB><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+    return hipGetLastError();
+}
+
+template <int MODE>
 hipError_t Launch<MODE>::gather_f(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
                                   float inv_scale, double *out, hipStream_t s) {
     const long long total = (long long)g.nx * g.ny * g.nz;
     if (lay == LAY_P) k_gather_f<MODE, LAY_P><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
-    else k_gather_f<MODE, LAY_Q><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
+    else if (lay == LAY_Q) k_gather_f<MODE, LAY_Q><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
+    else k_gather_f<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
     return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@ template <int MODE>
 struct Launch {
     // Fused collide-stream sweep.  odd: odd-type (LAY_P -> LAY_Q) else even-type.
     static hipError_t step(const StepArgs &a, bool odd, bool capture, dim3 block, hipStream_t s);
+    // Fused pull sweep of the two-array pattern (LAY_AB): a.pop -> a.pop_out.
+    static hipError_t step_ab(const StepArgs &a, bool capture, dim3 block, hipStream_t s);
     // Ghost layer of the open faces (NSCBC or rest state) written into the current layout.
     static hipError_t ghost(const GhostArgs &a, int lay, hipStream_t s);
     // Pull step of solid cells on open faces (their state feeds the NSCBC ghosts).
@@ -26,16 +28,24 @@ struct Launch {
                                   const double *rho, float scale, hipStream_t s);
     // Rest state of n tracked solid cells.
     static hipError_t fill_state_rest(void *state, int n, hipStream_t s);
+    // Wall copies of LAY_P (see k_step) after the state was set from outside.
+    static hipError_t prime_walls(void *pop, const uint8_t *mask, const GridDesc &g, int lay, double alpha,
+                                  hipStream_t s);
+    // Fill the x-face buffers (StepArgs::fb) from the lattice in layout lay.
+    static hipError_t fb_gather(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
+                                void *fb0, void *fb1, float scale, float inv_scale, hipStream_t s);
     // Gather f* of every interior cell into AoS FP64 full populations [cell][q] over the
     // interior, x fastest (for checkpoints and tests).
     static hipError_t gather_f(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
                                float inv_scale, double *out, hipStream_t s);
 };
 
-// Mode-independent helpers (kernels_common.hip).
+// Mode-independent helpers (voxel.hip).
 hipError_t launch_mask_flags(uint8_t *mask, const GridDesc &g, hipStream_t s);
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, long long n, uint8_t v, hipStream_t s);
+// Grid-stride copy of n16 16-byte words.
+hipError_t launch_copy16(const void *a, void *b, long long n16, hipStream_t s);
 
 // Voxeliser (voxel.hip).
 struct VoxelArgs {

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_mask_from_heights(VoxelArgs v, GridDesc
     const int i = (int)(t % (g.nx + 2));
     const int j = (int)((t / (g.nx + 2)) % (g.ny + 2));
     const int k = (int)(t / ((long long)(g.nx + 2) * (g.ny + 2)));
-    const long long c = (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    const long long c = cell_index(g, i, j, k);
     uint8_t m;
     if (k == 0) m = v.ground ? MK_SOLID : 0;
     else if (k == g.nz + 1 || i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1) m = 0;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_mask_flags(uint8_t *mask, GridDesc g) {
     const int i = (int)(n % g.nx) + 1;
     const int j = (int)((n / g.nx) % g.ny) + 1;
     const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
-    const long long c = (long long)i + (long long)g.px * ((long long)j + (long long)g.py * k);
+    const long long c = cell_index(g, i, j, k);
     uint8_t m = mask[c] & (uint8_t)~MK_NEAR;
     if (!(m & MK_SOLID)) {
         for (int q = 1; q < Q; ++q) {
@@ -133,6 +133,17 @@ __global__ __launch_bounds__(256) void k_fill_f32(float *p, long long n, float v
 __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, long long n, uint8_t v) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
+}
+
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *a, uint4 *b, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += st) b[t] = a[t];
+}
+
+hipError_t launch_copy16(const void *a, void *b, long long n16, hipStream_t s) {
+    k_copy16<<<256 * 64, 256, 0, s>>>(static_cast<const uint4 *>(a), static_cast<uint4 *>(b), n16);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s) {

@@ -17,7 +17,7 @@ from typing import Iterable, List, Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnlos_engine.so")
+LIB_PATH = os.environ.get("NLOS_ENGINE_LIB") or os.path.join(HERE, "libnlos_engine.so")
 
 F64, F32, F16S = 0, 1, 2
 PRECISIONS = {"f64": F64, "f32": F32, "f16s": F16S}
@@ -27,6 +27,8 @@ FACES_ALL = FACES_REFERENCE | FACE_ZM
 WAVE_GAUSS_PULSE, WAVE_SINE, WAVE_SAMPLED, WAVE_RICKER = 0, 1, 2, 3
 INJECT_GAUSS5, INJECT_TRILINEAR = 0, 1
 STEP_CAPTURE_FIELD = 1
+PATTERN_AUTO, PATTERN_INPLACE, PATTERN_TWO_ARRAY = 0, 1, 2
+PATTERNS = {"auto": PATTERN_AUTO, "inplace": PATTERN_INPLACE, "two_array": PATTERN_TWO_ARRAY}
 
 ERR_NAMES = {1: "NLOS_ERR_ARG", 2: "NLOS_ERR_HIP", 3: "NLOS_ERR_STATE", 4: "NLOS_ERR_NONFINITE",
              5: "NLOS_ERR_IO", 6: "NLOS_ERR_NOMEM", 7: "NLOS_ERR_UNSUPPORTED"}
@@ -42,7 +44,7 @@ class Config(C.Structure):
                 ("sponge_faces", C.c_int32), ("nscbc_faces", C.c_int32), ("ground", C.c_int32),
                 ("use_geometry", C.c_int32), ("output_world_coords", C.c_int32),
                 ("precision", C.c_int32), ("f16_scale_log2", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("stream_pattern", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Source(C.Structure):
@@ -57,7 +59,8 @@ class Source(C.Structure):
 
 class Info(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("precision", C.c_int32),
-                ("dt", C.c_double), ("step", C.c_int64), ("n_probes", C.c_int64),
+                ("dt", C.c_double), ("stream_pattern", C.c_int32), ("reserved_info", C.c_int32),
+                ("step", C.c_int64), ("n_probes", C.c_int64),
                 ("n_solid_boundary", C.c_int64), ("bytes_per_update", C.c_double),
                 ("device_bytes", C.c_size_t)]
 
@@ -101,6 +104,7 @@ SIGNATURES = {
     "nlos_timing_get": (C.c_int, [_E, C.POINTER(Timing)]),
     "nlos_debug_read_f": (C.c_int, [_E, _P]),
     "nlos_debug_poke": (C.c_int, [_E, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
+    "nlos_debug_copy_kernel": (C.c_int, [C.c_size_t, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -141,6 +145,8 @@ def default_config(**kw) -> Config:
     for k, v in kw.items():
         if k == "precision" and isinstance(v, str):
             v = PRECISIONS[v]
+        if k == "stream_pattern" and isinstance(v, str):
+            v = PATTERNS[v]
         if k == "world_offset":
             cfg.world_offset[:] = list(v)
             continue
@@ -341,6 +347,13 @@ class Engine:
 
     def debug_poke(self, i: int, j: int, k: int, q: int, f: float) -> None:
         _check(self.lib.nlos_debug_poke(self.h, i, j, k, q, f))
+
+
+def copy_kernel_ms(nbytes: int, reps: int = 5) -> float:
+    """Mean time of a plain device copy of nbytes (read nbytes, write nbytes)."""
+    ms = C.c_double()
+    _check(load_library().nlos_debug_copy_kernel(nbytes, reps, C.byref(ms)))
+    return ms.value
 
 
 def grid_dims(cfg: Config):

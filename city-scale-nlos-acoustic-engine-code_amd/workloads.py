@@ -126,3 +126,52 @@ class CityWorkload:
                 continue
             pts.append((x, y, 0.5 * self.dx))
         return np.array(pts)
+
+
+@dataclass
+class ScatterWorkload:
+    """Rigid-cube scattering: n^3 cells at dx = 1, tau = 0.5 + 1e-4, a 24^3 solid cube at
+    the centre ([52,76)^3 for n = 128), an initial Gaussian density pulse of amplitude
+    1e-3 rho0 and sigma 4 cells centred 30 cells off the cube centre along +x, a 16-cell
+    sponge ramping quadratically to 0.1 on all six faces, open faces with NSCBC on all six,
+    no ground, 8 probes at seeded positions in the fluid outside the sponge."""
+    n: int = 128
+    seed: int = 0
+    cube: int = 24
+    sponge_cells: int = 16
+    tau: float = 0.5 + 1e-4
+    n_probes: int = 8
+
+    def config(self, precision="f64") -> nlos.Config:
+        n = float(self.n)
+        return nlos.default_config(x_min=0.0, x_max=n, y_min=0.0, y_max=n, z_min=0.0, z_max=n,
+                                   dx=1.0, c_phys=343.0, tau=self.tau,
+                                   sponge_width=float(self.sponge_cells), sponge_strength=0.1,
+                                   sky_z=1e9, ground=0, sponge_faces=nlos.FACES_ALL,
+                                   nscbc_faces=nlos.FACES_ALL, precision=precision,
+                                   world_offset=(0.0, 0.0, 0.0))
+
+    def mask(self) -> np.ndarray:
+        n, h = self.n, self.cube // 2
+        m = np.zeros((n, n, n), np.uint8)
+        m[n // 2 - h:n // 2 + h, n // 2 - h:n // 2 + h, n // 2 - h:n // 2 + h] = 1
+        return m
+
+    def density(self) -> np.ndarray:
+        n = self.n
+        c = np.arange(n) + 0.5
+        z, y, x = np.meshgrid(c, c, c, indexing="ij")
+        cx, cy, cz = n / 2 + 30.0, n / 2, n / 2
+        r2 = (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2
+        return 1.0 + 1e-3 * np.exp(-r2 / (2.0 * 4.0 ** 2))
+
+    def probes(self) -> np.ndarray:
+        rng = np.random.Generator(np.random.PCG64(self.seed))
+        n, s, h = self.n, self.sponge_cells, self.cube // 2
+        pts = []
+        while len(pts) < self.n_probes:
+            p = rng.uniform(s, n - s, 3)
+            if np.all(np.abs(p - n / 2) < h + 1):
+                continue
+            pts.append(tuple(p))
+        return np.array(pts)

@@ -80,8 +80,18 @@ typedef struct nlos_config {
     int32_t output_world_coords; /* 05:255 */
     int32_t precision;         /* nlos_precision */
     int32_t f16_scale_log2;    /* NLOS_F16S: stored value = g * 2^k; 0 selects the default 24 */
-    int32_t reserved[5];
+    int32_t stream_pattern;    /* nlos_pattern */
+    int32_t reserved[4];
 } nlos_config;
+
+/* How populations stream.  Both give identical results; they differ in memory and speed.
+ * NLOS_PATTERN_INPLACE keeps one array (A-A: alternating own-cell and neighbour sweeps);
+ * NLOS_PATTERN_TWO_ARRAY keeps two (pull from one, write the other), which avoids the
+ * neighbour writes of the A-A odd sweep.  AUTO takes two arrays when both fit in free
+ * device memory. */
+typedef enum nlos_pattern {
+    NLOS_PATTERN_AUTO = 0, NLOS_PATTERN_INPLACE = 1, NLOS_PATTERN_TWO_ARRAY = 2
+} nlos_pattern;
 
 typedef enum nlos_waveform {
     NLOS_WAVE_GAUSS_PULSE = 0, /* amp*exp(-0.5*((t-t0)/sigma_t)^2)   05:579-581 */
@@ -115,6 +125,8 @@ typedef struct nlos_info {
     int32_t nx, ny, nz;        /* interior cells NXg, NYg, NZg (05:332-334) */
     int32_t precision;
     double dt;                 /* seconds (05:200-203) */
+    int32_t stream_pattern;    /* the pattern in use (NLOS_PATTERN_INPLACE or _TWO_ARRAY) */
+    int32_t reserved_info;
     int64_t step;              /* next time index s; t = s*dt (05:1079) */
     int64_t n_probes;
     int64_t n_solid_boundary;  /* solid cells on NSCBC faces (their state is tracked) */

@@ -20,6 +20,11 @@ int nlos_debug_read_f(nlos_engine *e, double *out);
  * population value f, converted to the engine's storage form. */
 int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, int32_t q, double f);
 
+/* Plain device copy of `bytes` (16-byte vector loads and stores, one read and one write
+ * per byte) repeated `reps` times on a private stream; *avg_ms is the mean launch time
+ * from HIP events.  Used to calibrate HBM byte counters and as the practical copy rate. */
+int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ def box_scene(precision="f64", nx=40, ny=32, nz=24, seed=3):
                               amp=2e-2, freq=120.0, waveform=nlos.WAVE_SINE)
     probes = [(3.5, 20.5, 1.5), (37.2, 15.3, 7.7), (20.0, 16.0, 20.9), (8.5, 8.5, 3.5),
               (33.5, 29.5, 10.5), (1.2, 30.8, 22.5)]
+    probes = [p for p in probes if p[0] < nx and p[1] < ny and p[2] < nz]
     return cfg, solid, src, probes
 
 

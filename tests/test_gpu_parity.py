@@ -54,11 +54,18 @@ def _oracle_interior_f(o):
     return o.read_f()[1:-1, 1:-1, 1:-1, :]
 
 
-@pytest.mark.parametrize("scene", ["box", "pulse", "reflect"])
-def test_f64_bit_identical_to_oracle(gpu, scene):
-    print(f"[f64 vs oracle] scene={scene}", flush=True)
+PATTERNS = ["inplace", "two_array"]
+
+
+@pytest.mark.parametrize("pattern", PATTERNS)
+@pytest.mark.parametrize("scene", ["box", "box64", "pulse", "reflect"])
+def test_f64_bit_identical_to_oracle(gpu, scene, pattern):
+    print(f"[f64 vs oracle] scene={scene} pattern={pattern}", flush=True)
     if scene == "box":
         cfg, solid, src, probes = scenes.box_scene("f64")
+        rho, steps = None, 60
+    elif scene == "box64":      # nx a multiple of 64: the odd sweep's lane-shift path
+        cfg, solid, src, probes = scenes.box_scene("f64", nx=64, ny=32, nz=20)
         rho, steps = None, 60
     elif scene == "pulse":
         cfg, solid, rho, probes = scenes.pulse_scene("f64")
@@ -66,7 +73,9 @@ def test_f64_bit_identical_to_oracle(gpu, scene):
     else:
         cfg, solid, src, probes = scenes.reflect_scene("f64")
         rho, steps = None, 50
+    cfg.stream_pattern = nlos.PATTERNS[pattern]
     e = _engine_run(cfg, solid, src, probes, steps, rho)
+    assert e.info().stream_pattern == nlos.PATTERNS[pattern]
     o = _oracle_run(cfg, solid, src, probes, steps, rho)
     got, want = e.read_probes(), o.read_probes()
     assert got.shape == want.shape
@@ -79,12 +88,14 @@ def test_f64_bit_identical_to_oracle(gpu, scene):
     e.close()
 
 
+@pytest.mark.parametrize("pattern", PATTERNS)
 @pytest.mark.parametrize("precision,tol", [("f32", TOL_F32), ("f16s", TOL_F16)])
 @pytest.mark.parametrize("scene", ["box", "reflect"])
-def test_reduced_precision_vs_f64(gpu, precision, tol, scene):
-    print(f"[{precision} vs f64] scene={scene}", flush=True)
+def test_reduced_precision_vs_f64(gpu, precision, tol, scene, pattern):
+    print(f"[{precision} vs f64] scene={scene} pattern={pattern}", flush=True)
     make = scenes.box_scene if scene == "box" else scenes.reflect_scene
     cfg, solid, src, probes = make(precision)
+    cfg.stream_pattern = nlos.PATTERNS[pattern]
     e = _engine_run(cfg, solid, src, probes, 80)
     cfg64, _, _, _ = make("f64")
     o = _oracle_run(cfg64, solid, src, probes, 80)
@@ -95,8 +106,10 @@ def test_reduced_precision_vs_f64(gpu, precision, tol, scene):
     e.close()
 
 
-def test_three_phase_api_equals_fused_step(gpu):
+@pytest.mark.parametrize("pattern", PATTERNS)
+def test_three_phase_api_equals_fused_step(gpu, pattern):
     cfg, solid, src, probes = scenes.box_scene("f64")
+    cfg.stream_pattern = nlos.PATTERNS[pattern]
     a = _engine_run(cfg, solid, src, probes, 25)
     b = nlos.Engine(cfg)
     b.set_mask(solid)
@@ -112,9 +125,11 @@ def test_three_phase_api_equals_fused_step(gpu):
     assert np.array_equal(a.debug_read_f()[solid == 0], b.debug_read_f()[solid == 0])
 
 
+@pytest.mark.parametrize("pattern", PATTERNS)
 @pytest.mark.parametrize("precision", ["f32", "f16s"])
-def test_rest_state_is_exact_fixed_point(gpu, precision):
+def test_rest_state_is_exact_fixed_point(gpu, precision, pattern):
     cfg, solid, _, probes = scenes.box_scene(precision)
+    cfg.stream_pattern = nlos.PATTERNS[pattern]
     e = nlos.Engine(cfg)
     e.set_mask(solid)
     e.set_probes(probes)
@@ -127,9 +142,11 @@ def test_rest_state_is_exact_fixed_point(gpu, precision):
     assert np.array_equal(f, np.broadcast_to(w, f.shape))
 
 
+@pytest.mark.parametrize("pattern", PATTERNS)
 @pytest.mark.parametrize("precision", ["f64", "f16s"])
-def test_checkpoint_restore_is_bitwise(gpu, precision):
+def test_checkpoint_restore_is_bitwise(gpu, precision, pattern):
     cfg, solid, src, probes = scenes.box_scene(precision)
+    cfg.stream_pattern = nlos.PATTERNS[pattern]
     e = _engine_run(cfg, solid, src, probes, 17)
     blob = e.checkpoint()
     e.step(20)
@@ -172,9 +189,11 @@ def test_voxelizer_matches_oracle_bitwise(gpu):
     assert e.info().n_solid_boundary > 0
 
 
-def _reference05_engine(scene):
+def _reference05_engine(scene, pattern="auto"):
     import city_stl
-    e = nlos.Engine(nlos.reference05_config())
+    cfg = nlos.reference05_config()
+    cfg.stream_pattern = nlos.PATTERNS[pattern]
+    e = nlos.Engine(cfg)
     if scene == "city":
         e.voxelize_stl(city_stl.reference_city_stl(), 1.0, -22248.0, -4123.0, -14.0)
     else:
@@ -185,11 +204,12 @@ def _reference05_engine(scene):
     return e
 
 
+@pytest.mark.parametrize("pattern", PATTERNS)
 @pytest.mark.parametrize("scene", ["flat", "city"])
-def test_f64_reproduces_reference_probe_files(gpu, scene, tmp_path):
+def test_f64_reproduces_reference_probe_files(gpu, scene, pattern, tmp_path):
     """05_sensor_illusion's own run (393+1 steps) compiled from the reference sources."""
-    print(f"[reference goldens] scene={scene}", flush=True)
-    e = _reference05_engine(scene)
+    print(f"[reference goldens] scene={scene} pattern={pattern}", flush=True)
+    e = _reference05_engine(scene, pattern)
     frames = json.load(open(os.path.join(GOLDEN, f"ref05_{scene}", "vtk_frames.json")))
     total = int(round(1.0 / e.dt))
     assert total == 393
@@ -247,3 +267,19 @@ def test_cpp_lbmsolver_program_reproduces_reference(gpu, scene, tmp_path):
         got = (tmp_path / f"probe_{n}.txt").read_text()
         want = open(os.path.join(GOLDEN, f"ref05_{scene}", f"probe_{n}.txt")).read()
         assert got == want, n
+
+
+@pytest.mark.parametrize("precision", ["f64", "f16s"])
+def test_lane_shift_path_equals_plain_loads(gpu, precision, monkeypatch):
+    """The odd sweep's aligned-load + lane-shift path (full 64-cell rows) gives the same
+    bits as plain shifted loads."""
+    cfg, solid, src, probes = scenes.box_scene(precision, nx=128, ny=24, nz=16)
+    cfg.stream_pattern = nlos.PATTERN_INPLACE
+    runs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("NLOS_DPP", flag)
+        e = _engine_run(cfg, solid, src, probes, 21)
+        runs.append((e.read_probes(), e.debug_read_f()))
+        e.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1], equal_nan=True)
