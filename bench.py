@@ -31,7 +31,22 @@ import nlos_engine as nlos  # noqa: E402
 import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# Measured HBM bytes per sweep launch (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE,
+# calibrated on a copy kernel of known size) for the default workload; see the file.
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_step_ab_f16s_city1024x1024x256_traffic.md")
 METRIC = "MLUPS & % of HBM roofline, city grid: FP16-store vs FP64 at stated probe error"
+
+
+def measured_traffic():
+    """traffic_bytes_per_launch from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg: str) -> None:
@@ -152,6 +167,10 @@ def main() -> None:
     bpu = head["bytes_per_update"]
     alg_bytes = fluid * bpu + head["solid"] * 1.0     # per sweep: 19 loads + 19 stores + mask
     achieved = alg_bytes / (head["kernel_avg_ms"] * 1e-3) / 1e9
+    # The counter summary covers the default workload's FP16S two-array sweep only.
+    default_wl = (args.nx, args.ny, args.nz) == (1024, 1024, 256)
+    traffic = (measured_traffic() if default_wl and head["precision"] == "f16s"
+               and head["pattern"] == PATTERN_NAMES[2] else None)
     out = {
         "metric": METRIC,
         "value": round(head["mlups"], 1),
@@ -172,7 +191,7 @@ def main() -> None:
                    "cells": head["cells"], "solid_cells": head["solid"],
                    "stream_pattern": head["pattern"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_update": bpu, "kernel_avg_ms": round(head["kernel_avg_ms"], 4)},
     }
     for p, r in res.items():

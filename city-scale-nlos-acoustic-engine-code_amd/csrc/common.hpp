@@ -65,8 +65,8 @@ __host__ __device__ __forceinline__ long long cell_index(const GridDesc &g, int 
 // swept; in LAY_P the slot q of a wall cell w holds the reflected f*_q(w - e_q) of its
 // fluid neighbour, so that the odd sweep reads bounced populations like streamed ones.
 //   LAY_AB (two arrays, pull sweep): slot q of x in the current array; the other array
-//   receives the next step.  In LAY_AB the slot INV[q] of a wall cell w holds the reflected
-//   f*_q(w - e_q) of its fluid neighbour, read by the next pull like a streamed population.
+//   receives the next step.  Bounce-back reads the cell's own slot INV[q]; the slots of
+//   solid cells hold the rest state and are never read.
 enum Layout : int { LAY_P = 0, LAY_Q = 1, LAY_AB = 2 };
 
 struct StepArgs {
@@ -81,13 +81,17 @@ struct StepArgs {
     const long long *probe_cells;        // sorted unique probe cells
     int n_probe_cells;
     double *probe_row;                   // this step's row of the probe history
+    const long long *step_ctr;           // if set: the row is hist + *step_ctr * n_probe_cells
+    double *hist;
     float *field;                        // interior rho-1 (capture sweeps only)
     int *flag;                           // bit 0: non-finite density seen
     int tiles_x, tiles_y;                // block tiling of the interior
     int nblocks;
+    int kc;                              // two-array sweep: planes per workgroup column
     int xcd_remap;                       // map consecutive tiles to one XCD
     int dpp;                             // every lane active: odd sweep may shift across lanes
     void *fb[2];                         // x-face buffers (x-, x+) or null, see below
+    const uint32_t *wall_bits;           // two-array sweep: bit q set when c - e_q is solid
 };
 
 // x-face buffers.  The cells I = (1, j, k) and (nx, j, k) feed the NSCBC ghosts of the x
@@ -131,6 +135,17 @@ struct SourceArgs {
     int n;
     double alpha;                         // wall reflection (for the wall copies)
     float scale, inv_scale;
+};
+
+// Source entries of several scenes over a run of steps, read at the step the device counter
+// names: entries step_off[t] .. step_off[t+1] of step t = *ctr - *ctr_base.
+struct SourceMulti {
+    const SourceArgs *scenes;
+    const long long *ctr, *ctr_base;
+    const long long *step_off;
+    const int *entry_scene;
+    const long long *cells;
+    const double *adds;
 };
 
 }  // namespace nlos

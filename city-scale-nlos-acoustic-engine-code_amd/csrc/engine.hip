@@ -146,7 +146,7 @@ struct nlos_engine {
     int device = 0;
 
     DevBuf pop, pop2, mask, sig[3], flag, field, hist, probe_cells, s_a, s_b, solid_cells, src_cells,
-        src_adds, face_map[6], gh, vox_tri, vox_bins, vox_start, rho_tmp, fb[2];
+        src_adds, face_map[6], gh, vox_tri, vox_bins, vox_start, rho_tmp, fb[2], wall_bits;
     bool fb_valid = false;           // x-face buffers agree with the lattice
     bool walls_valid = false;        // LAY_P wall copies agree with the lattice
     bool field_valid = false;
@@ -169,6 +169,7 @@ struct nlos_engine {
     dim3 block{64, 4, 1};
     int xcd_remap = 0;
     int use_dpp = 1;
+    int col_planes = 16;             // two-array sweep: planes per workgroup column
 
     bool timing = false;
     std::vector<hipEvent_t> ev;     // start/stop pairs
@@ -246,6 +247,7 @@ int finalize_mask(nlos_engine *e) {
     e->fb_valid = false;
     e->walls_valid = false;
     HIP_TRY(launch_mask_flags(e->mask.as<uint8_t>(), g, e->stream));
+    if (e->wall_bits.p) HIP_TRY(launch_wall_bits(e->mask.as<uint8_t>(), g, e->wall_bits.as<uint32_t>(), e->stream));
     for (long long c : e->probe_ucells) {
         uint8_t b = 0;
         HIP_TRY(hipMemcpyAsync(&b, e->mask.as<uint8_t>() + c, 1, hipMemcpyDeviceToHost, e->stream));
@@ -538,15 +540,17 @@ int launch_sweep(nlos_engine *e, bool capture) {
     a.flag = e->flag.as<int>();
     a.tiles_x = (e->g.nx + (int)e->block.x - 1) / (int)e->block.x;
     a.tiles_y = (e->g.ny + (int)e->block.y - 1) / (int)e->block.y;
-    a.nblocks = a.tiles_x * a.tiles_y * e->g.nz;
+    a.kc = e->pattern == NLOS_PATTERN_TWO_ARRAY ? std::max(1, std::min(e->col_planes, e->g.nz)) : 1;
+    a.nblocks = a.tiles_x * a.tiles_y * ((e->g.nz + a.kc - 1) / a.kc);
     a.xcd_remap = e->xcd_remap;
     // lane shifts need full waves: rows of exactly 64-cell tiles, no partial block rows
     a.dpp = e->use_dpp && e->block.x == 64 && e->g.nx % 64 == 0 && e->g.ny % (int)e->block.y == 0;
     a.fb[0] = e->fb[0].p;
     a.fb[1] = e->fb[1].p;
+    a.wall_bits = e->wall_bits.as<uint32_t>();
     const bool two = e->pattern == NLOS_PATTERN_TWO_ARRAY;
     const bool odd = e->lay == LAY_P;
-    if ((two || odd) && !e->walls_valid)
+    if (!two && odd && !e->walls_valid)
         HIP_TRY(by_mode(e->mode, [&](auto L) {
             return decltype(L)::prime_walls(e->curp(), e->mask.as<uint8_t>(), e->g, e->lay, e->cfg.wall_reflect,
                                             e->stream);
@@ -678,9 +682,10 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         e->scale = std::ldexp(1.0f, k);
         e->inv_scale = std::ldexp(1.0f, -k);
     }
-    if (const char *b = std::getenv("NLOS_BLOCK")) parse_block(b, e->block);
+    const char *block_env = std::getenv("NLOS_BLOCK");
     if (const char *x = std::getenv("NLOS_XCD_REMAP")) e->xcd_remap = std::atoi(x) != 0;
     if (const char *x = std::getenv("NLOS_DPP")) e->use_dpp = std::atoi(x) != 0;
+    if (const char *x = std::getenv("NLOS_COLUMN")) e->col_planes = std::max(1, std::atoi(x));
 
     auto bail = [&](int rc) { nlos_destroy(e); return rc; };
     hipError_t he = hipGetDevice(&e->device);
@@ -693,16 +698,26 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         size_t freeb = 0, totalb = 0;
         if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = 0;
         // two arrays when both fit with room for the mask and work buffers
-        const size_t need = 2 * popb + (size_t)e->g.ncell + ((size_t)4 << 30);
+        const size_t need = 2 * popb + 5 * (size_t)e->g.ncell + ((size_t)4 << 30);
         e->pattern = freeb >= need ? NLOS_PATTERN_TWO_ARRAY : NLOS_PATTERN_INPLACE;
     }
     if (e->pattern != NLOS_PATTERN_INPLACE && e->pattern != NLOS_PATTERN_TWO_ARRAY)
         return bail(fail(NLOS_ERR_ARG, "bad stream_pattern %d", cfg->stream_pattern));
     e->lay = e->pattern == NLOS_PATTERN_TWO_ARRAY ? LAY_AB : LAY_P;
+    // Measured on the 1024x1024x256 city: the pull sweep runs fastest on 256-cell row
+    // segments walking 4 planes, the in-place sweep on 64x4 tiles (its lane shifts need
+    // 64-cell rows).
+    if (e->pattern == NLOS_PATTERN_TWO_ARRAY) {
+        e->block = dim3(256, 1, 1);
+        if (!std::getenv("NLOS_COLUMN")) e->col_planes = 4;
+    }
+    if (block_env) parse_block(block_env, e->block);
     if (e->pop.reserve(popb) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate %.2f GB of populations", popb / 1e9));
     if (e->pattern == NLOS_PATTERN_TWO_ARRAY && e->pop2.reserve(popb) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate a second %.2f GB population array", popb / 1e9));
+    if (e->pattern == NLOS_PATTERN_TWO_ARRAY && e->wall_bits.reserve((size_t)e->g.ncell * 4) != hipSuccess)
+        return bail(fail(NLOS_ERR_NOMEM, "cannot allocate wall bits"));
     if (e->mask.reserve((size_t)e->g.ncell) != hipSuccess || e->flag.reserve(sizeof(int)) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
     for (int f = 0; f < 2; ++f)
@@ -740,7 +755,7 @@ NLOS_API int nlos_destroy(nlos_engine *e) {
                       &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
                       &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
                       &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
-                      &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1]};
+                      &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1], &e->wall_bits};
     for (DevBuf *b : bufs) b->release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -763,7 +778,7 @@ NLOS_API int nlos_get_info(const nlos_engine *e, nlos_info *info) {
     size_t tot = 0;
     const DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                             &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
-                            &e->src_adds, &e->gh};
+                            &e->src_adds, &e->gh, &e->fb[0], &e->fb[1], &e->wall_bits};
     for (const DevBuf *b : bufs) tot += b->bytes;
     info->device_bytes = tot;
     return NLOS_OK;

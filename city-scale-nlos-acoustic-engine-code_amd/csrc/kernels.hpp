@@ -137,34 +137,48 @@ __device__ __forceinline__ float geq32(int q, float dr, float r, float ux, float
 
 // BGK collision in deviation form with the sponge as a factor (1 - sigma) on g
 // ((1-s)f + s*w - w = (1-s)g).  Works on the stored values G = g*S (S = 1 for FP32, 2^k
-// for FP16): the moments are scaled back once, the equilibrium is formed in stored units,
-// and a direction and its inverse share the even part of the equilibrium
-// (cu_inv = -cu).  Returns drho.
+// for FP16): the moments are scaled back once and the equilibrium is formed in stored
+// units.  Ordering B pairs every direction with its inverse; a pair shares the even part
+// of the equilibrium (cu_inv = -cu), and its sum and difference give the moments.
+//   g_post = keep*((1-omega)*g + omega*geq),  geq = w*(drho + rho*(3cu + 4.5cu^2 - 1.5u^2))
+// Returns drho.
 __device__ __forceinline__ float collide32(float *G, float omega, float keep, float S, float invS) {
-    float drs, jx, jy, jz;
-    moments32(G, drs, jx, jy, jz);
+    constexpr int PAIR[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+    float sp[9], dp[9];
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+        sp[n] = G[PAIR[n]] + G[INV[PAIR[n]]];
+        dp[n] = G[PAIR[n]] - G[INV[PAIR[n]]];
+    }
+    // pairs: 1 (1,0,0) 3 (0,1,0) 5 (0,0,1) 7 (1,1,0) 8 (1,-1,0) 11 (1,0,1) 12 (1,0,-1)
+    //        15 (0,1,1) 16 (0,1,-1)
+    float drs = G[0] + (((sp[0] + sp[1]) + (sp[2] + sp[3])) + ((sp[4] + sp[5]) + (sp[6] + sp[7])) + sp[8]);
+    const float jx = ((dp[0] + dp[3]) + (dp[4] + dp[5])) + dp[6];
+    const float jy = ((dp[1] + dp[3]) + (dp[7] + dp[8])) - dp[4];
+    const float jz = ((dp[2] + dp[5]) + (dp[7] - dp[6])) - dp[8];
     float dr = drs * invS;
     float r = 1.0f + dr;
     if (r <= 0.0f) { r = 1e-12f; dr = r - 1.0f; drs = dr * S; }
-    const float k = invS / r;
+    const float k = invS * __builtin_amdgcn_rcpf(r);
     const float ux = jx * k, uy = jy * k, uz = jz * k;
     const float u2c = 1.5f * (ux * ux + uy * uy + uz * uz);
     const float rS = r * S;
-    const float om1 = 1.0f - omega;
-    {
-        const float ge = wq<float>(0) * (drs - rS * u2c);
-        G[0] = keep * (om1 * G[0] + omega * ge);
-    }
-    constexpr int PAIR[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};   // one of each inverse pair
+    const float a = keep * (1.0f - omega);      // weight of the old population
+    const float bw = keep * omega;               // weight of the equilibrium
+    G[0] = a * G[0] + (bw * wq<float>(0)) * (drs - rS * u2c);
+    // per weight class (1/18: pairs 0-2, 1/36: pairs 3-8)
+    const float c1a = bw * wq<float>(1) * drs, c2a = bw * wq<float>(1) * rS, c3a = 3.0f * c2a;
+    const float c1b = bw * wq<float>(7) * drs, c2b = bw * wq<float>(7) * rS, c3b = 3.0f * c2b;
 #pragma unroll
     for (int n = 0; n < 9; ++n) {
         const int q = PAIR[n], p = INV[q];
+        const float c1 = n < 3 ? c1a : c1b, c2 = n < 3 ? c2a : c2b, c3 = n < 3 ? c3a : c3b;
         const float cu = cu_of<float>(q, ux, uy, uz);
-        const float wr = wq<float>(q) * rS;
-        const float even = wq<float>(q) * drs + wr * (4.5f * cu * cu - u2c);
-        const float odd = wr * (3.0f * cu);
-        G[q] = keep * (om1 * G[q] + omega * (even + odd));
-        G[p] = keep * (om1 * G[p] + omega * (even - odd));
+        const float x = (4.5f * cu) * cu - u2c;
+        const float even = c2 * x + c1;
+        const float odd = c3 * cu;
+        G[q] = a * G[q] + (even + odd);
+        G[p] = a * G[p] + (even - odd);
     }
     return dr;
 }
@@ -175,6 +189,16 @@ template <int MODE>
 __device__ __forceinline__ typename Tr<MODE>::C reflect(int q, typename Tr<MODE>::C v, double alpha) {
     if constexpr (MODE == M_F64) return W64[q] + alpha * (v - W64[q]);
     else return static_cast<float>(alpha) * v;
+}
+
+// Several scenes in one launch: thread t belongs to the scene s with start[s] <= t < start[s+1].
+__device__ __forceinline__ int scene_of(const long long *start, int n, long long t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
 }
 
 // ---- fused collide-stream sweep ---------------------------------------------------------
@@ -218,17 +242,34 @@ __device__ __forceinline__ void st_stream(S *p, S v) {
 #endif
 }
 
+template <class S> __device__ __forceinline__ S ld_b(const S *base, unsigned tb);
+template <class S> __device__ __forceinline__ void st_b(S *base, unsigned tb, S v);
+
+// Lane access base[t] through a wave-uniform base (see ld_b / st_b).  `base` must be the
+// same for every active lane; per-lane addresses use ld / st.
 template <int MODE>
 __device__ __forceinline__ typename Tr<MODE>::C ldu(const typename Tr<MODE>::S *base, unsigned t, float inv_scale) {
-    if constexpr (MODE == M_F16) return static_cast<float>(ld_stream(base + t)) * inv_scale;
-    else return ld_stream(base + t);
+    using S = typename Tr<MODE>::S;
+#ifdef NLOS_NONTEMPORAL
+    const S v = ld_stream(base + t);
+#else
+    const S v = ld_b(base, t * (unsigned)sizeof(S));
+#endif
+    if constexpr (MODE == M_F16) return static_cast<float>(v) * inv_scale;
+    else return v;
 }
 
 template <int MODE>
 __device__ __forceinline__ void stu(typename Tr<MODE>::S *base, unsigned t, typename Tr<MODE>::C v, float scale) {
     using S = typename Tr<MODE>::S;
-    if constexpr (MODE == M_F16) st_stream(base + t, static_cast<S>(v * scale));
-    else st_stream(base + t, v);
+    S x;
+    if constexpr (MODE == M_F16) x = static_cast<S>(v * scale);
+    else x = v;
+#ifdef NLOS_NONTEMPORAL
+    st_stream(base + t, x);
+#else
+    st_b(base, t * (unsigned)sizeof(S), x);
+#endif
 }
 
 // Whole-wave shift by one lane: with right = true lane L receives lane L-1's value and lane
@@ -361,7 +402,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
             if ((face == 0 ? i != 1 : i != a.g.nx) || !a.fb[face]) continue;
             S *fb = static_cast<S *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) stu<MODE>(fb, q, f[q], RAW);
+            for (int q = 0; q < Q; ++q) st<MODE>(fb, q, f[q], RAW);   // per-lane address
         }
     }
 
@@ -392,107 +433,244 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 
 // Byte-offset access through a wave-uniform base: the address is an SGPR base plus a
 // 32-bit VGPR offset (global_load ... saddr), with no per-lane 64-bit arithmetic.
+// The base goes through readfirstlane: it must be wave-uniform (the same for every active
+// lane), and the barrier keeps the compiler from folding it into a per-lane 64-bit address.
+// (Global address space is kept explicit so that the accesses stay global_load/store.)
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 template <class S>
 __device__ __forceinline__ S ld_b(const S *base, unsigned tb) {
-    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + tb);
+    using GC = __attribute__((address_space(1))) const char;
+    using GS = __attribute__((address_space(1))) const S;
+    GC *b = (GC *)uniform_u64(reinterpret_cast<unsigned long long>(base));
+    return *(GS *)(b + tb);
 }
 
 template <class S>
 __device__ __forceinline__ void st_b(S *base, unsigned tb, S v) {
-    *reinterpret_cast<S *>(reinterpret_cast<char *>(base) + tb) = v;
+    using GC = __attribute__((address_space(1))) char;
+    using GS = __attribute__((address_space(1))) S;
+    GC *b = (GC *)uniform_u64(reinterpret_cast<unsigned long long>(base));
+    *(GS *)(b + tb) = v;
 }
 
-// Pull sweep of the two-array pattern (LAY_AB, 05:917-968): f_in_q(c) = cur[q][c - e_q]
-// (the wall copy when c - e_q is solid), collide, write nxt[q][c] (aligned, local); a cell
-// next to a wall also writes the reflected f*_q(c) into slot INV[q] of the wall cell
-// c + e_q of nxt.
+// Pull sweep of the two-array pattern (LAY_AB, 05:917-968): f_in_q(c) = cur[q][c - e_q],
+// or the half-way bounce-back cur[INV q][c] when c - e_q is solid; collide; write
+// nxt[q][c] (aligned, local).  Solid lanes store the rest state into their own slots,
+// which nothing reads: every wave then writes whole 128-byte lines, where a line with
+// unwritten lanes costs the memory a read-modify-write.
+//
+// A workgroup walks a column of a.kc planes of its (x, y) tile and issues the loads of
+// plane k+1 before it collides plane k, so that its reads stay in flight while it computes.
+
+template <class S>
+__device__ __forceinline__ S read_lane(S v, int src) {
+    if constexpr (sizeof(S) == 2) {
+        const int b = __builtin_bit_cast(unsigned short, v);
+        return __builtin_bit_cast(S, (unsigned short)__builtin_amdgcn_readlane(b, src));
+    } else if constexpr (sizeof(S) == 4) {
+        return __builtin_bit_cast(S, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, src);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), src);
+        return __builtin_bit_cast(S, (long long)(((unsigned long long)hi << 32) | lo));
+    }
+}
+
+// rec[q] = f[q] of lane src, written by lanes 0..Q-1 with one store instruction (a single
+// lane storing Q scattered values costs the sweep far more).  Wave-uniform call; lanes
+// 0..Q-1 must be active.
+template <class S, class C>
+__device__ __forceinline__ void store_record(S *rec, const C *f, int src, int lane) {
+    S mine = S(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const S r = read_lane(static_cast<S>(f[q]), src);
+        if (lane == q) mine = r;
+    }
+    if (lane < Q) rec[lane] = mine;
+}
+
+// Stored populations and mask byte of one cell (the loads of one plane of a column walk).
+// FP16 values are held two to a register (the loads fill the low and high halves), which
+// halves the registers that the prefetched plane occupies.
+template <int MODE>
+struct AbIn {
+    typename Tr<MODE>::S v[Q];
+    uint8_t m;
+    __device__ __forceinline__ typename Tr<MODE>::C get(int q) const { return static_cast<typename Tr<MODE>::C>(v[q]); }
+};
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+template <>
+struct AbIn<M_F16> {
+    half2_t v[(Q + 1) / 2];
+    uint8_t m;
+    __device__ __forceinline__ float get(int q) const {
+        return static_cast<float>((q & 1) ? v[q >> 1].y : v[q >> 1].x);
+    }
+};
+
+template <int MODE>
+__device__ __forceinline__ void ab_load(AbIn<MODE> &in, const StepArgs &a, long long cb, unsigned t, unsigned tb) {
+    using S = typename Tr<MODE>::S;
+    const S *cur = static_cast<const S *>(a.pop);
+    const long long N = a.g.ncell;
+    in.m = (a.mask + cb)[t];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const S v = ld_b(cur + ((long long)q * N + cb - nb_off(a.g, q)), tb);
+        if constexpr (MODE == M_F16) {
+            if (q & 1) in.v[q >> 1].y = v;
+            else in.v[q >> 1].x = v;
+        } else {
+            in.v[q] = v;
+        }
+    }
+    if constexpr (MODE == M_F16) in.v[Q >> 1].y = _Float16(0);
+}
+
 template <int MODE, bool CAPTURE>
-__global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
+__device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in, typename Tr<MODE>::C sig,
+                                        int i, int j, int k, long long cb, unsigned t, unsigned tb) {
     using C = typename Tr<MODE>::C;
     using S = typename Tr<MODE>::S;
-    int b = blockIdx.x;
-    if (a.xcd_remap) b = remap_block(b, gridDim.x);
+    const long long N = a.g.ncell;
+    const long long c = cb + t;
+    const uint8_t m = in.m;
+    C f[Q];
+    if (m & MK_SOLID) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            if constexpr (MODE == M_F64) f[q] = W64[q];
+            else f[q] = 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = in.get(q);
+        if (m & MK_NEAR) {
+            const uint32_t sb = a.wall_bits[c];   // bit q: c - e_q is solid
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                if ((sb >> q) & 1u) {
+                    C v = ld<MODE>(a.pop, (long long)INV[q] * N + c, 1.0f);
+                    if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
+                    f[q] = v;
+                }
+            }
+        }
+
+        double dval;   // rho - 1
+        bool finite;
+        if constexpr (MODE == M_F64) {
+            const double r = collide64(f, a.omega, sig);
+            dval = r - 1.0;
+            finite = isfinite(r);
+        } else {
+            const float dr = collide32(f, (float)a.omega, 1.0f - sig, a.scale, a.inv_scale);
+            dval = dr;
+            finite = isfinite(dr);
+        }
+        if (!finite) atomicOr(a.flag, 1);
+        if (m & MK_PROBE) {
+            const int u = probe_slot(a.probe_cells, a.n_probe_cells, c);
+            double *row = a.step_ctr ? a.hist + (*a.step_ctr) * a.n_probe_cells : a.probe_row;
+            if (u >= 0) row[u] = dval;
+        }
+        if constexpr (CAPTURE) {
+            a.field[(long long)(i - 1) + (long long)a.g.nx * ((long long)(j - 1) + (long long)a.g.ny * (k - 1))] =
+                (float)dval;
+        }
+    }
+    // x-face records of the cells (1, j, k) and (nx, j, k) when fluid (see StepArgs::fb);
+    // every test below is uniform over the wave (one row segment of 64 cells).
+    const int lane = (int)(threadIdx.x & 63u);
+    const int iw = i - lane;   // first cell of this wave
+#pragma unroll
+    for (int face = 0; face < 2; ++face) {
+        if (!a.fb[face]) continue;
+        if (face == 0 ? iw != 1 : !(iw <= a.g.nx && a.g.nx < iw + 64)) continue;
+        const int src = face == 0 ? 0 : a.g.nx - iw;
+        if (__builtin_amdgcn_readlane((int)m, src) & MK_SOLID) continue;
+        S *rec = static_cast<S *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
+        if (a.g.nx - iw >= Q - 1) {
+            store_record(rec, f, src, lane);
+        } else if (lane == src) {   // fewer than Q active lanes in this row segment
+#pragma unroll
+            for (int q = 0; q < Q; ++q) rec[q] = static_cast<S>(f[q]);
+        }
+    }
+    S *nxt = static_cast<S *>(a.pop_out);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) st_b(nxt + ((long long)q * N + cb), tb, static_cast<S>(f[q]));
+}
+
+// Column b: tile (bi, bj) of the x-y plane, planes k0 .. k0 + a.kc - 1 (clipped to nz).
+template <int MODE, bool CAPTURE>
+__device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
+    using C = typename Tr<MODE>::C;
+    using S = typename Tr<MODE>::S;
     const int bi = b % a.tiles_x;
     const int rest_b = b / a.tiles_x;
     const int bj = rest_b % a.tiles_y;
-    const int k = rest_b / a.tiles_y + 1;
+    const int k0 = (rest_b / a.tiles_y) * a.kc + 1;
+    const int k1 = min(k0 + a.kc, a.g.nz + 1);
     const int i0 = bi * (int)blockDim.x + 1;
     const int j0 = bj * (int)blockDim.y + 1;
     const int i = i0 + (int)threadIdx.x;
     const int j = j0 + (int)threadIdx.y;
     if (i > a.g.nx || j > a.g.ny) return;
 
-    const long long N = a.g.ncell;
-    const long long cb = cell_index(a.g, i0, j0, k);                  // block corner, uniform
+    long long cb = cell_index(a.g, i0, j0, k0);                       // tile corner, uniform
     const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;  // lane offset (cells)
     const unsigned tb = t * (unsigned)sizeof(S);                      // lane offset (bytes)
-    const long long c = cb + t;
-    const S *cur = static_cast<const S *>(a.pop);
-    S *nxt = static_cast<S *>(a.pop_out);
-    const C *sx = static_cast<const C *>(a.sig_x);
-    const C *sy = static_cast<const C *>(a.sig_y);
     const C *sz = static_cast<const C *>(a.sig_z);
+    // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
+    const C sxy = fmax(static_cast<const C *>(a.sig_x)[i], static_cast<const C *>(a.sig_y)[j]);
 
-    // All loads first; the addresses are inside the lattice for any interior cell.
-    const uint8_t m = (a.mask + cb)[t];
-    C f[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const S v = ld_b(cur + ((long long)q * N + cb - nb_off(a.g, q)), tb);
-        if constexpr (MODE == M_F16) f[q] = static_cast<float>(v);
-        else f[q] = v;
-    }
-    const C sig = fmax(sx[i], fmax(sy[j], sz[k]));
-    if (m & MK_SOLID) return;
-
-    double dval;   // rho - 1
-    bool finite;
-    if constexpr (MODE == M_F64) {
-        const double r = collide64(f, a.omega, sig);
-        dval = r - 1.0;
-        finite = isfinite(r);
-    } else {
-        const float dr = collide32(f, (float)a.omega, 1.0f - sig, a.scale, a.inv_scale);
-        dval = dr;
-        finite = isfinite(dr);
-    }
-    if (!finite) atomicOr(a.flag, 1);
-    if (m & MK_PROBE) {
-        const int u = probe_slot(a.probe_cells, a.n_probe_cells, c);
-        if (u >= 0) a.probe_row[u] = dval;
-    }
-    if constexpr (CAPTURE) {
-        a.field[(long long)(i - 1) + (long long)a.g.nx * ((long long)(j - 1) + (long long)a.g.ny * (k - 1))] =
-            (float)dval;
-    }
-    if (i == 1 || i == a.g.nx) {
-#pragma unroll
-        for (int face = 0; face < 2; ++face) {
-            if ((face == 0 ? i != 1 : i != a.g.nx) || !a.fb[face]) continue;
-            S *fb = static_cast<S *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
-#pragma unroll
-            for (int q = 0; q < Q; ++q) fb[q] = static_cast<S>(f[q]);
+    AbIn<MODE> in;
+    ab_load<MODE>(in, a, cb, t, tb);
+    for (int k = k0;; ++k) {
+        if (k + 1 >= k1) {
+            ab_cell<MODE, CAPTURE>(a, in, fmax(sxy, sz[k]), i, j, k, cb, t, tb);
+            break;
         }
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) st_b(nxt + ((long long)q * N + cb), tb, static_cast<S>(f[q]));
-    if (m & MK_NEAR) {
-#pragma unroll
-        for (int q = 1; q < Q; ++q) {
-            const long long w = c + nb_off(a.g, q);
-            if (a.mask[w] & MK_SOLID) {
-                C v = f[q];
-                if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
-                nxt[(long long)INV[q] * N + w] = static_cast<S>(v);
-            }
-        }
+        AbIn<MODE> nx_in;
+        ab_load<MODE>(nx_in, a, cb + a.g.plane, t, tb);
+        ab_cell<MODE, CAPTURE>(a, in, fmax(sxy, sz[k]), i, j, k, cb, t, tb);
+        in = nx_in;
+        cb += a.g.plane;
     }
 }
 
-// Wall copies of LAY_P and LAY_AB (see Layout): for every fluid cell c next to a wall and
-// every q with c + e_q solid, the wall cell receives the reflected f*_q(c).  Needed after
-// the state was set from outside.
-template <int MODE, int LAY>
+template <int MODE, bool CAPTURE>
+__global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
+    int b = blockIdx.x;
+    if (a.xcd_remap) b = remap_block(b, gridDim.x);
+    step_ab_body<MODE, CAPTURE>(a, b);
+}
+
+// Persistent sweep over the tiles of several scenes: each workgroup walks the global tile
+// range with stride gridDim.x and finds its scene by the tile prefix.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_step_ab_multi(const StepArgs *scenes, const long long *start, int n) {
+    const long long total = start[n];
+    for (long long b = blockIdx.x; b < total; b += gridDim.x) {
+        const int s = scene_of(start, n, b);
+        step_ab_body<MODE, false>(scenes[s], (int)(b - start[s]));
+    }
+}
+
+// Wall copies of LAY_P (see Layout): for every fluid cell c next to a wall and every q
+// with c + e_q solid, the wall cell receives the reflected f*_q(c) (held in slot INV[q] of
+// c) in its slot q.  Needed after the state was set from outside.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_prime_walls(void *pop, const uint8_t *mask, GridDesc g, double alpha) {
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)g.nx * g.ny * g.nz;
@@ -507,11 +685,9 @@ __global__ __launch_bounds__(256) void k_prime_walls(void *pop, const uint8_t *m
     for (int q = 1; q < Q; ++q) {
         const long long t = c + nb_off(g, q);
         if (!(mask[t] & MK_SOLID)) continue;
-        // LAY_P: f*_q(c) in slot INV[q] of c, copy to slot q of the wall;
-        // LAY_AB: f*_q(c) in slot q of c, copy to slot INV[q] of the wall
-        auto v = ld<MODE>(pop, (long long)(LAY == LAY_AB ? q : INV[q]) * N + c, 1.0f);
+        auto v = ld<MODE>(pop, (long long)INV[q] * N + c, 1.0f);
         if (alpha != 1.0) v = reflect<MODE>(q, v, alpha);
-        st<MODE>(pop, (long long)(LAY == LAY_AB ? INV[q] : q) * N + t, v, 1.0f);
+        st<MODE>(pop, (long long)q * N + t, v, 1.0f);
     }
 }
 
@@ -536,9 +712,8 @@ __device__ __forceinline__ typename Tr<MODE>::C read_post(const void *pop, const
 // ---- ghost layer (05:671-904 at one rank) --------------------------------------------------
 
 template <int MODE, int LAY>
-__global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
+__device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
     using C = typename Tr<MODE>::C;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n_halo) return;
     const GridDesc &g = a.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -591,7 +766,7 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
             const typename Tr<MODE>::S *fb = static_cast<const typename Tr<MODE>::S *>(a.fb[face & 1]) +
                                              ((long long)(jj - 1) + (long long)ny * (kk - 1)) * Q;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) fI[q] = ldu<MODE>(fb, q, a.inv_scale);
+            for (int q = 0; q < Q; ++q) fI[q] = ld<MODE>(fb, q, a.inv_scale);   // per-thread address
         } else if (a.mask[I] & MK_SOLID) {
             int pidx;
             if (axis == 0) pidx = (jj - 1) + ny * (kk - 1);
@@ -686,14 +861,26 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
     }
 }
 
+template <int MODE, int LAY>
+__global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
+    ghost_body<MODE, LAY>(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int MODE, int LAY>
+__global__ __launch_bounds__(256) void k_ghost_multi(const GhostArgs *scenes, const long long *start, int n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= start[n]) return;
+    const int s = scene_of(start, n, t);
+    ghost_body<MODE, LAY>(scenes[s], t - start[s]);
+}
+
 // ---- solid cells on open faces (05:917-938 for a SOLID dst) -------------------------------
 // f_next[b][q] = solid(b - e_q) ? f[b][inv q] : f*_q(b - e_q).  With b solid, f*_q of a
 // fluid neighbour sits in slot INV[q] of that neighbour in either A-A layout and in slot q
 // of it in LAY_AB; the ghost kernel puts a halo neighbour's value in the same place.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_solid(SolidArgs a) {
+__device__ __forceinline__ void solid_body(const SolidArgs &a, int n) {
     using C = typename Tr<MODE>::C;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.n) return;
     const long long c = a.cells[n];
     const long long N = a.g.ncell;
@@ -713,12 +900,22 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs a) {
     }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(256) void k_solid(SolidArgs a) {
+    solid_body<MODE>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_solid_multi(const SolidArgs *scenes, const long long *start, int n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= start[n]) return;
+    const int s = scene_of(start, n, t);
+    solid_body<MODE>(scenes[s], (int)(t - start[s]));
+}
+
 // ---- source injection (05:660-665) ---------------------------------------------------------
 template <int MODE, int LAY>
-__global__ __launch_bounds__(256) void k_source(SourceArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n * Q) return;
-    const int e = t / Q, q = t % Q;
+__device__ __forceinline__ void source_body(const SourceArgs &a, int e, int q) {
     const long long c = a.cells[e];
     const long long N = a.g.ncell;
     long long idx;
@@ -739,12 +936,11 @@ __global__ __launch_bounds__(256) void k_source(SourceArgs a) {
         v = ld<MODE>(a.pop, idx, a.inv_scale) + (float)(W64[q] * a.adds[e]);
         st<MODE>(a.pop, idx, v, a.scale);
     }
-    if constexpr (LAY != LAY_Q) {
+    if constexpr (LAY == LAY_P) {
         // keep the wall copy of a population headed into a wall in step (see Layout)
         const long long tc = c + nb_off(a.g, q);
         if (q != 0 && (a.mask[tc] & MK_SOLID))
-            st<MODE>(a.pop, (long long)(LAY == LAY_AB ? INV[q] : q) * N + tc,
-                     a.alpha != 1.0 ? reflect<MODE>(q, v, a.alpha) : v, a.scale);
+            st<MODE>(a.pop, (long long)q * N + tc, a.alpha != 1.0 ? reflect<MODE>(q, v, a.alpha) : v, a.scale);
     }
     // the x-face buffer holds the same f*_q of boundary cells; apply the same add to it
     const long long r = c - a.g.origin;
@@ -763,6 +959,28 @@ __global__ __launch_bounds__(256) void k_source(SourceArgs a) {
             }
         }
     }
+}
+
+template <int MODE, int LAY>
+__global__ __launch_bounds__(256) void k_source(SourceArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n * Q) return;
+    source_body<MODE, LAY>(a, t / Q, t % Q);
+}
+
+// All scenes' source entries of the current step (index ctr - ctr_base into step_off).
+template <int MODE, int LAY>
+__global__ __launch_bounds__(256) void k_source_multi(SourceMulti m) {
+    const long long st = *m.ctr - *m.ctr_base;
+    const long long e0 = m.step_off[st], e1 = m.step_off[st + 1];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (e1 - e0) * Q) return;
+    const long long e = e0 + t / Q;
+    SourceArgs a = m.scenes[m.entry_scene[e]];
+    a.cells = m.cells + e;
+    a.adds = m.adds + e;
+    a.n = 1;
+    source_body<MODE, LAY>(a, 0, (int)(t % Q));
 }
 
 // Fills the x-face buffers from the lattice (after the state was set from outside).
@@ -920,8 +1138,8 @@ template <int MODE>
 hipError_t Launch<MODE>::prime_walls(void *pop, const uint8_t *mask, const GridDesc &g, int lay, double alpha,
                                      hipStream_t s) {
     const long long total = (long long)g.nx * g.ny * g.nz;
-    if (lay == LAY_AB) k_prime_walls<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, alpha);
-    else k_prime_walls<MODE, LAY_P><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, alpha);
+    if (lay != LAY_P) return hipErrorInvalidValue;
+    k_prime_walls<MODE><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, alpha);
     return hipGetLastError();
 }
 

@@ -44,6 +44,10 @@ struct Launch {
 hipError_t launch_mask_flags(uint8_t *mask, const GridDesc &g, hipStream_t s);
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, long long n, uint8_t v, hipStream_t s);
+// Per interior cell: bit q set when c - e_q is solid (two-array bounce-back).
+hipError_t launch_wall_bits(const uint8_t *mask, const GridDesc &g, uint32_t *bits, hipStream_t s);
+// *ctr += by on the device (one thread).
+hipError_t launch_count(long long *ctr, long long by, hipStream_t s);
 // Grid-stride copy of n16 16-byte words.
 hipError_t launch_copy16(const void *a, void *b, long long n16, hipStream_t s);
 

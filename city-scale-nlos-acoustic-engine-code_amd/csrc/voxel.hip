@@ -125,6 +125,32 @@ hipError_t launch_mask_flags(uint8_t *mask, const GridDesc &g, hipStream_t s) {
     return hipGetLastError();
 }
 
+// bits[c], interior fluid cells: bit q set when the pull source c - e_q is solid (the
+// two-array sweep bounces those populations back); 0 for solid cells.
+__global__ __launch_bounds__(256) void k_wall_bits(const uint8_t *mask, GridDesc g, uint32_t *bits) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    if (n >= total) return;
+    const int i = (int)(n % g.nx) + 1;
+    const int j = (int)((n / g.nx) % g.ny) + 1;
+    const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
+    const long long c = cell_index(g, i, j, k);
+    uint32_t b = 0;
+    if (!(mask[c] & MK_SOLID)) {
+        for (int q = 1; q < Q; ++q) {
+            const long long o = (long long)EX[q] + (long long)EY[q] * g.px + (long long)EZ[q] * g.plane;
+            if (mask[c - o] & MK_SOLID) b |= 1u << q;
+        }
+    }
+    bits[c] = b;
+}
+
+hipError_t launch_wall_bits(const uint8_t *mask, const GridDesc &g, uint32_t *bits, hipStream_t s) {
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    k_wall_bits<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(mask, g, bits);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void k_fill_f32(float *p, long long n, float v) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -143,6 +169,14 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *a, uint4 *b, long l
 
 hipError_t launch_copy16(const void *a, void *b, long long n16, hipStream_t s) {
     k_copy16<<<256 * 64, 256, 0, s>>>(static_cast<const uint4 *>(a), static_cast<uint4 *>(b), n16);
+    return hipGetLastError();
+}
+
+// Advances a device step counter (replayed graphs index their step by it).
+__global__ void k_count(long long *ctr, long long by) { *ctr += by; }
+
+hipError_t launch_count(long long *ctr, long long by, hipStream_t s) {
+    k_count<<<1, 1, 0, s>>>(ctr, by);
     return hipGetLastError();
 }
 

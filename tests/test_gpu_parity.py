@@ -58,7 +58,7 @@ PATTERNS = ["inplace", "two_array"]
 
 
 @pytest.mark.parametrize("pattern", PATTERNS)
-@pytest.mark.parametrize("scene", ["box", "box64", "pulse", "reflect"])
+@pytest.mark.parametrize("scene", ["box", "box64", "narrow", "wide", "pulse", "reflect"])
 def test_f64_bit_identical_to_oracle(gpu, scene, pattern):
     print(f"[f64 vs oracle] scene={scene} pattern={pattern}", flush=True)
     if scene == "box":
@@ -67,6 +67,12 @@ def test_f64_bit_identical_to_oracle(gpu, scene, pattern):
     elif scene == "box64":      # nx a multiple of 64: the odd sweep's lane-shift path
         cfg, solid, src, probes = scenes.box_scene("f64", nx=64, ny=32, nz=20)
         rho, steps = None, 60
+    elif scene == "narrow":     # rows shorter than 19 cells: x-face records from one lane
+        cfg, solid, src, probes = scenes.box_scene("f64", nx=12, ny=20, nz=14)
+        rho, steps = None, 40
+    elif scene == "wide":       # a partial last row segment, columns clipped at nz
+        cfg, solid, src, probes = scenes.box_scene("f64", nx=100, ny=10, nz=37)
+        rho, steps = None, 40
     elif scene == "pulse":
         cfg, solid, rho, probes = scenes.pulse_scene("f64")
         src, steps = None, 40
@@ -283,3 +289,21 @@ def test_lane_shift_path_equals_plain_loads(gpu, precision, monkeypatch):
         e.close()
     assert np.array_equal(runs[0][0], runs[1][0])
     assert np.array_equal(runs[0][1], runs[1][1], equal_nan=True)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f16s"])
+def test_two_array_launch_shapes_agree(gpu, precision, monkeypatch):
+    """The two-array sweep gives the same bits for every workgroup shape and column length
+    (planes walked per workgroup, with prefetch of the next plane)."""
+    cfg, solid, src, probes = scenes.box_scene(precision, nx=100, ny=12, nz=37)
+    cfg.stream_pattern = nlos.PATTERN_TWO_ARRAY
+    runs = []
+    for block, col in (("64x4", "1"), ("64x4", "16"), ("256x1", "4"), ("128x2", "64")):
+        monkeypatch.setenv("NLOS_BLOCK", block)
+        monkeypatch.setenv("NLOS_COLUMN", col)
+        e = _engine_run(cfg, solid, src, probes, 23)
+        runs.append((e.read_probes(), e.debug_read_f()))
+        e.close()
+    for r in runs[1:]:
+        assert np.array_equal(runs[0][0], r[0])
+        assert np.array_equal(runs[0][1], r[1], equal_nan=True)
