@@ -125,6 +125,108 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a - b))
 
 
+def bench_scatter(args) -> None:
+    """Rigid-cube scattering, 128^3, FP64, 2000 steps: GPU MLUPS, and the probe traces
+    against the oracle over the same steps (the oracle run is the CPU baseline)."""
+    from oracle_ffi import Oracle
+    wl = workloads.ScatterWorkload()
+    steps = 2000
+    cfg = wl.config("f64")
+    cfg.stream_pattern = nlos.PATTERNS[args.pattern]
+    mask, rho, probes = wl.mask(), wl.density(), wl.probes()
+    e = nlos.Engine(cfg)
+    e.set_mask(mask)
+    e.set_density(rho)
+    e.set_probes(probes)
+    e.step(1)          # first launches outside the timed region
+    e.synchronize()
+    e.timing(True)
+    e.timing_reset()
+    t0 = time.perf_counter()
+    e.step(steps - 1)
+    e.synchronize()
+    wall = time.perf_counter() - t0
+    tm = e.timing(False)
+    got = e.read_probes()
+    info = e.info()
+    e.close()
+    cells = wl.n ** 3
+    mlups = cells * (steps - 1) / wall / 1e6
+    log(f"scatter GPU f64: {mlups:.0f} MLUPS, kernel {tm.avg_ms:.4f} ms")
+    out = {"metric": "MLUPS, FP64 rigid-cube scattering (probe traces against the oracle)",
+           "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": 1,
+           "ms_per_step": round(1e3 * wall / (steps - 1), 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "rigid-cube scattering 128^3, dx=1, tau=0.5+1e-4, 24^3 cube, "
+                                  "Gaussian pulse 1e-3 sigma 4 at +30 cells, 16-cell sponge on 6 "
+                                  "faces, 8 probes", "stream_pattern": PATTERN_NAMES.get(info.stream_pattern)},
+           "kernel_avg_ms": round(tm.avg_ms, 4)}
+    if not args.no_cpu:
+        log("oracle run (2000 steps) ...")
+        o = Oracle(cfg)
+        o.set_mask(mask)
+        o.set_density(rho)
+        o.set_probes(probes)
+        t0 = time.perf_counter()
+        o.run(steps)
+        el = time.perf_counter() - t0
+        want = o.read_probes()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        out["probe_rel_l2_vs_oracle"] = rel_l2(got, want)
+        out["probe_bit_identical"] = bool(np.array_equal(got, want))
+        out["cpu_baseline"] = {"value": cells * steps / el / 1e6, "unit": "MLUPS", "cores": threads,
+                               "kind": "port", "sample": f"oracle FP64 two-array OpenMP, full 128^3 "
+                                                         f"case, {steps} steps in {el:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
+def bench_city2048(args) -> None:
+    """Precision sweep at dx = 0.5 m: FP16S and FP32 MLUPS on the 2048x2048x512 city (FP64
+    does not fit), then per-probe error of FP16S/FP32 against FP64 on a 512x512x256 sub-box
+    of the same construction (all three precisions on identical geometry and source)."""
+    steps = args.steps if args.steps != 200 else 40
+    warmup = min(args.warmup, 5)
+    wl = workloads.CityWorkload(nx=2048, ny=2048, nz=512, dx=0.5, sponge_cells=64, n_probes=128)
+    t = time.perf_counter()
+    stl = wl.stl()
+    log(f"city2048: {len(wl.boxes)} boxes ({time.perf_counter() - t:.1f} s)")
+    res = {}
+    for p in ("f16s", "f32"):
+        log(f"running {p}: warmup {warmup}, timed {steps}")
+        r = run_precision(wl, stl, p, warmup, steps, pattern=args.pattern)
+        r.pop("mask")
+        res[p] = r
+        log(f"  {p}: {r['mlups']:.0f} MLUPS, kernel {r['kernel_avg_ms']:.3f} ms ({r['pattern']})")
+    out = {"metric": "MLUPS and probe error, dx=0.5 m precision sweep (2048x2048x512)",
+           "value": round(res["f16s"]["mlups"], 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": round(res["f16s"]["ms_per_step"], 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": f"moving-source city 2048x2048x512, dx=0.5 m, {len(wl.boxes)} "
+                                  f"seeded boxes, 128 ground probes"}}
+    for p, r in res.items():
+        cells, solid = r["cells"], r["solid"]
+        alg = (cells - solid) * r["bytes_per_update"] + solid
+        out[p] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_avg_ms"], 4),
+                  "pattern": r["pattern"], "bytes_per_update": r["bytes_per_update"],
+                  "hbm_frac": round(alg / (r["kernel_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if args.error_steps > 0:
+        n_err = args.error_steps if args.error_steps != 600 else 4000
+        sub = workloads.CityWorkload(nx=512, ny=512, nz=256, dx=0.5, sponge_cells=64, n_probes=128)
+        sstl = sub.stl()
+        traces = {}
+        for p in ("f64", "f32", "f16s"):
+            log(f"sub-box error run {p}: {n_err} steps")
+            traces[p] = run_precision(sub, sstl, p, 0, n_err, timing=False)["probes"]
+        ref = traces["f64"]
+        err = {"steps": n_err, "sub_box": "512x512x256 at dx=0.5 m", "signal_max": float(np.abs(ref).max())}
+        for p in ("f32", "f16s"):
+            per = [rel_l2(a, b) for a, b in zip(traces[p], ref) if np.abs(b).max() > 0]
+            err[p] = {"rel_l2": rel_l2(traces[p], ref), "per_probe_rel_l2": [round(x, 6) for x in per]}
+        out["probe_error_vs_f64"] = err
+    print(json.dumps(out), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,7 +241,11 @@ def main() -> None:
     ap.add_argument("--only", default="f16s,f32,f64")
     ap.add_argument("--pattern", default="auto", choices=["auto", "inplace", "two_array"])
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a plain 1 GiB device copy kernel (PMC byte calibration)")
+                    help="also run a plain 2 GiB device copy kernel (PMC byte calibration)")
+    ap.add_argument("--workload", default="city", choices=["city", "scatter", "city2048"],
+                    help="city: the headline moving-source city step (default); scatter: the "
+                         "128^3 rigid-cube case, FP64 against the oracle; city2048: the dx=0.5 m "
+                         "precision sweep on 2048x2048x512 with its 512x512x256 error sub-box")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -147,6 +253,10 @@ def main() -> None:
         sys.exit(2)
     if not os.path.exists(nlos.LIB_PATH):
         raise SystemExit("engine library not built; run __graft_entry__.build()")
+    if args.workload == "scatter":
+        return bench_scatter(args)
+    if args.workload == "city2048":
+        return bench_city2048(args)
 
     wl = workloads.CityWorkload(nx=args.nx, ny=args.ny, nz=args.nz)
     t = time.perf_counter()
