@@ -165,6 +165,7 @@ struct nlos_engine {
     nlos_source src{};
     std::vector<double> samples;
     bool has_source = false;
+    nlos_batch *batch = nullptr;     // set when the engine is a scene of a batch
 
     dim3 block{64, 4, 1};
     int xcd_remap = 0;
@@ -449,21 +450,98 @@ int upload_schedule(nlos_engine *e) {
 // ---- launch helpers -------------------------------------------------------------------
 int sync_face_buffers(nlos_engine *e);
 
-int launch_source(nlos_engine *e, long long off, long long n) {
-    if (n <= 0) return NLOS_OK;
-    if (int rc = sync_face_buffers(e)) return rc;
+// Kernel arguments for the state `phase` steps ahead of the current one (0 = current).
+// A two-array step swaps the population arrays, and a step with tracked boundary solids
+// swaps their two state buffers; phase 1 is the state after one such step.
+void *pop_at(const nlos_engine *e, int phase) { return ((e->cur ^ phase) & 1) ? e->pop2.p : e->pop.p; }
+void *solid_state_at(const nlos_engine *e, int phase) { return (phase & 1) ? e->s_b.p : e->s_a.p; }
+long long halo_cells(const GridDesc &g) {
+    const long long nx = g.nx, ny = g.ny, nz = g.nz;
+    return 2 * (nx + 2) * (ny + 2) + 2 * (nx + 2) * nz + 2 * ny * nz;
+}
+
+SourceArgs source_args(const nlos_engine *e, int phase) {
     SourceArgs a{};
-    a.pop = e->curp();
+    a.pop = e->pattern == NLOS_PATTERN_TWO_ARRAY ? pop_at(e, phase) : e->pop.p;
     a.fb[0] = e->fb[0].p;
     a.fb[1] = e->fb[1].p;
     a.mask = e->mask.as<uint8_t>();
     a.g = e->g;
-    a.cells = e->src_cells.as<long long>() + off;
-    a.adds = e->src_adds.as<double>() + off;
-    a.n = (int)n;
     a.alpha = e->cfg.wall_reflect;
     a.scale = e->scale;
     a.inv_scale = e->inv_scale;
+    return a;
+}
+
+GhostArgs ghost_args(const nlos_engine *e, int phase) {
+    GhostArgs a{};
+    a.pop = e->pattern == NLOS_PATTERN_TWO_ARRAY ? pop_at(e, phase) : e->pop.p;
+    a.fb[0] = e->fb[0].p;
+    a.fb[1] = e->fb[1].p;
+    a.mask = e->mask.as<uint8_t>();
+    a.g = e->g;
+    a.nscbc_faces = e->cfg.nscbc_faces;
+    a.s_old = solid_state_at(e, phase);
+    for (int f = 0; f < 6; ++f) a.face_map[f] = e->face_map[f].as<int>();
+    a.scale = e->scale;
+    a.inv_scale = e->inv_scale;
+    a.n_halo = halo_cells(e->g);
+    return a;
+}
+
+SolidArgs solid_args(const nlos_engine *e, int phase) {
+    SolidArgs s{};
+    s.pop = e->pattern == NLOS_PATTERN_TWO_ARRAY ? pop_at(e, phase) : e->pop.p;
+    s.mask = e->mask.as<uint8_t>();
+    s.g = e->g;
+    s.lay = e->lay;
+    s.cells = e->solid_cells.as<long long>();
+    s.n = e->n_solid;
+    s.s_old = solid_state_at(e, phase);
+    s.s_new = solid_state_at(e, phase + 1);
+    s.alpha = e->cfg.wall_reflect;
+    s.scale = e->scale;
+    s.inv_scale = e->inv_scale;
+    return s;
+}
+
+StepArgs step_args(const nlos_engine *e, int phase, dim3 block, int col_planes) {
+    StepArgs a{};
+    const bool two = e->pattern == NLOS_PATTERN_TWO_ARRAY;
+    a.pop = two ? pop_at(e, phase) : e->pop.p;
+    a.pop_out = two ? pop_at(e, phase + 1) : nullptr;
+    a.mask = e->mask.as<uint8_t>();
+    a.sig_x = e->sig[0].p;
+    a.sig_y = e->sig[1].p;
+    a.sig_z = e->sig[2].p;
+    a.g = e->g;
+    a.omega = e->omega;
+    a.alpha = e->cfg.wall_reflect;
+    a.scale = e->scale;
+    a.inv_scale = e->inv_scale;
+    a.probe_cells = e->probe_cells.as<long long>();
+    a.n_probe_cells = (int)e->probe_ucells.size();
+    a.flag = e->flag.as<int>();
+    a.tiles_x = (e->g.nx + (int)block.x - 1) / (int)block.x;
+    a.tiles_y = (e->g.ny + (int)block.y - 1) / (int)block.y;
+    a.kc = two ? std::max(1, std::min(col_planes, e->g.nz)) : 1;
+    a.nblocks = a.tiles_x * a.tiles_y * ((e->g.nz + a.kc - 1) / a.kc);
+    a.xcd_remap = e->xcd_remap;
+    // lane shifts need full waves: rows of exactly 64-cell tiles, no partial block rows
+    a.dpp = e->use_dpp && block.x == 64 && e->g.nx % 64 == 0 && e->g.ny % (int)block.y == 0;
+    a.fb[0] = e->fb[0].p;
+    a.fb[1] = e->fb[1].p;
+    a.wall_bits = e->wall_bits.as<uint32_t>();
+    return a;
+}
+
+int launch_source(nlos_engine *e, long long off, long long n) {
+    if (n <= 0) return NLOS_OK;
+    if (int rc = sync_face_buffers(e)) return rc;
+    SourceArgs a = source_args(e, 0);
+    a.cells = e->src_cells.as<long long>() + off;
+    a.adds = e->src_adds.as<double>() + off;
+    a.n = (int)n;
     HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::source(a, e->lay, e->stream); }));
     return NLOS_OK;
 }
@@ -480,33 +558,10 @@ int sync_face_buffers(nlos_engine *e) {
 
 int launch_halos(nlos_engine *e) {
     if (int rc = sync_face_buffers(e)) return rc;
-    GhostArgs a{};
-    a.pop = e->curp();
-    a.fb[0] = e->fb[0].p;
-    a.fb[1] = e->fb[1].p;
-    a.mask = e->mask.as<uint8_t>();
-    a.g = e->g;
-    a.nscbc_faces = e->cfg.nscbc_faces;
-    a.s_old = e->s_a.p;
-    for (int f = 0; f < 6; ++f) a.face_map[f] = e->face_map[f].as<int>();
-    a.scale = e->scale;
-    a.inv_scale = e->inv_scale;
-    const long long nx = e->g.nx, ny = e->g.ny, nz = e->g.nz;
-    a.n_halo = 2 * (nx + 2) * (ny + 2) + 2 * (nx + 2) * nz + 2 * ny * nz;
+    const GhostArgs a = ghost_args(e, 0);
     HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::ghost(a, e->lay, e->stream); }));
     if (e->n_solid > 0) {
-        SolidArgs s{};
-        s.pop = e->curp();
-        s.mask = e->mask.as<uint8_t>();
-        s.g = e->g;
-        s.lay = e->lay;
-        s.cells = e->solid_cells.as<long long>();
-        s.n = e->n_solid;
-        s.s_old = e->s_a.p;
-        s.s_new = e->s_b.p;
-        s.alpha = e->cfg.wall_reflect;
-        s.scale = e->scale;
-        s.inv_scale = e->inv_scale;
+        const SolidArgs s = solid_args(e, 0);
         HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::solid(s, e->stream); }));
         std::swap(e->s_a, e->s_b);
     }
@@ -521,33 +576,9 @@ int launch_sweep(nlos_engine *e, bool capture) {
         }
         HIP_TRY(launch_fill_f32(e->field.as<float>(), e->n_interior(), 0.0f, e->stream));
     }
-    StepArgs a{};
-    a.pop = e->curp();
-    a.pop_out = e->nxtp();
-    a.mask = e->mask.as<uint8_t>();
-    a.sig_x = e->sig[0].p;
-    a.sig_y = e->sig[1].p;
-    a.sig_z = e->sig[2].p;
-    a.g = e->g;
-    a.omega = e->omega;
-    a.alpha = e->cfg.wall_reflect;
-    a.scale = e->scale;
-    a.inv_scale = e->inv_scale;
-    a.probe_cells = e->probe_cells.as<long long>();
-    a.n_probe_cells = (int)e->probe_ucells.size();
+    StepArgs a = step_args(e, 0, e->block, e->col_planes);
     a.probe_row = a.n_probe_cells ? e->hist.as<double>() + (size_t)e->step * a.n_probe_cells : nullptr;
     a.field = capture ? e->field.as<float>() : nullptr;
-    a.flag = e->flag.as<int>();
-    a.tiles_x = (e->g.nx + (int)e->block.x - 1) / (int)e->block.x;
-    a.tiles_y = (e->g.ny + (int)e->block.y - 1) / (int)e->block.y;
-    a.kc = e->pattern == NLOS_PATTERN_TWO_ARRAY ? std::max(1, std::min(e->col_planes, e->g.nz)) : 1;
-    a.nblocks = a.tiles_x * a.tiles_y * ((e->g.nz + a.kc - 1) / a.kc);
-    a.xcd_remap = e->xcd_remap;
-    // lane shifts need full waves: rows of exactly 64-cell tiles, no partial block rows
-    a.dpp = e->use_dpp && e->block.x == 64 && e->g.nx % 64 == 0 && e->g.ny % (int)e->block.y == 0;
-    a.fb[0] = e->fb[0].p;
-    a.fb[1] = e->fb[1].p;
-    a.wall_bits = e->wall_bits.as<uint32_t>();
     const bool two = e->pattern == NLOS_PATTERN_TWO_ARRAY;
     const bool odd = e->lay == LAY_P;
     if (!two && odd && !e->walls_valid)
@@ -750,6 +781,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
 
 NLOS_API int nlos_destroy(nlos_engine *e) {
     if (!e) return NLOS_OK;
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; destroy the batch");
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                       &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
@@ -952,6 +984,7 @@ NLOS_API int nlos_set_probes(nlos_engine *e, const double *xyz, int32_t n, uint8
 // ---- stepping -----------------------------------------------------------------------------
 NLOS_API int nlos_apply_source(nlos_engine *e, double t) {
     if (int rc = ready(e)) return rc;
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
     e->sched_cells.clear();
     e->sched_adds.clear();
     schedule_step(e, t);
@@ -962,11 +995,13 @@ NLOS_API int nlos_apply_source(nlos_engine *e, double t) {
 
 NLOS_API int nlos_exchange_halos(nlos_engine *e) {
     if (int rc = ready(e)) return rc;
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
     return launch_halos(e);
 }
 
 NLOS_API int nlos_step_collision(nlos_engine *e) {
     if (int rc = ready(e)) return rc;
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
     int rc = ensure_hist(e, e->step + 1);
     if (rc) return rc;
     return launch_sweep(e, true);
@@ -974,6 +1009,7 @@ NLOS_API int nlos_step_collision(nlos_engine *e) {
 
 NLOS_API int nlos_step(nlos_engine *e, int64_t n, int32_t flags) {
     if (int rc = ready(e)) return rc;
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
     if (n < 0) return fail(NLOS_ERR_ARG, "negative step count");
     if (n == 0) return NLOS_OK;
     int rc = ensure_hist(e, e->step + n);
@@ -1111,6 +1147,7 @@ NLOS_API int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *need
 
 NLOS_API int nlos_restore(nlos_engine *e, const void *buf, size_t len) {
     if (int rc = ready(e)) return rc;
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; it cannot be restored alone");
     if (!buf || len < sizeof(CkptHeader)) return fail(NLOS_ERR_ARG, "checkpoint too short");
     CkptHeader h;
     std::memcpy(&h, buf, sizeof h);
@@ -1227,5 +1264,417 @@ NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) 
     (void)hipStreamDestroy(s);
     a.release();
     b.release();
+    return NLOS_OK;
+}
+
+// =========================================================================================
+// Batched scenes.  Every scene is an ordinary engine (two-array pattern); the batch owns
+// them and advances them together with multi-scene launches: per step one source launch
+// (entries of all scenes, indexed by a device step counter), one ghost and one boundary
+// solid launch over the concatenated halo / solid ranges, one persistent sweep over the
+// concatenated tile ranges, and a counter increment.  Kernel arguments of every scene for
+// the current state (phase 0) and the state one step later (phase 1) sit in device arrays,
+// so a captured graph of two steps is replayed for every pair of steps.
+struct nlos_batch {
+    std::vector<nlos_engine *> scenes;
+    int mode = M_F64;
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    dim3 block{256, 1, 1};
+    int col_planes = 4;
+    DevBuf step_args[2], ghost_args[2], solid_args[2], source_args[2];
+    DevBuf tile_start, halo_start, solid_start, ctr;
+    long long total_tiles = 0, total_halo = 0, total_solid = 0, max_entries = 0;
+    unsigned sweep_grid = 0;
+    DevBuf sched_off, sched_scene, sched_cells, sched_adds;
+    long long h_ctr[2] = {0, 0};          // host staging of the counter and its run base
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<long long> baked;         // launch shape the graph was captured with
+    long long step = 0;
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double timed_ms = 0.0;
+    long long timed_steps = 0;
+};
+
+namespace {
+
+int batch_ready(const nlos_batch *b) {
+    if (!b) return fail(NLOS_ERR_ARG, "null batch");
+    return NLOS_OK;
+}
+
+long long stencil_bound(const nlos_engine *e) {
+    if (!e->has_source) return 0;
+    if (e->src.injection == NLOS_INJECT_TRILINEAR) return 8;
+    const long long d = 2LL * e->src.stencil_radius + 1;
+    return d * d * d;
+}
+
+void drop_graph(nlos_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Upload a host vector into a device buffer of at least its size; `moved` is set when the
+// buffer had to be reallocated (its address is baked into a captured graph).
+template <class T>
+int upload_vec(nlos_batch *b, DevBuf &d, const std::vector<T> &v, bool &moved) {
+    const size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
+    if (n > d.bytes) {
+        HIP_TRY(d.reserve(std::max(n, d.bytes * 2)));
+        moved = true;
+    }
+    if (!v.empty()) HIP_TRY(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, b->stream));
+    return NLOS_OK;
+}
+
+// Per-scene argument arrays of both phases and the prefix arrays of tiles, halo cells and
+// tracked solids, for the current scene states and launch shape.
+int batch_prepare(nlos_batch *b, bool &moved) {
+    const int n = (int)b->scenes.size();
+    std::vector<StepArgs> st[2];
+    std::vector<GhostArgs> gh[2];
+    std::vector<SolidArgs> so[2];
+    std::vector<SourceArgs> sr[2];
+    std::vector<long long> tiles(1, 0), halo(1, 0), solid(1, 0);
+    b->max_entries = 0;
+    for (nlos_engine *e : b->scenes) {
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (int rc = sync_face_buffers(e)) return rc;
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        for (int ph = 0; ph < 2; ++ph) {
+            StepArgs a = step_args(e, ph, b->block, b->col_planes);
+            a.step_ctr = b->ctr.as<long long>();
+            a.hist = e->hist.as<double>();
+            if (!a.hist) a.n_probe_cells = 0;
+            st[ph].push_back(a);
+            gh[ph].push_back(ghost_args(e, ph));
+            so[ph].push_back(solid_args(e, ph));
+            sr[ph].push_back(source_args(e, ph));
+        }
+        tiles.push_back(tiles.back() + st[0].back().nblocks);
+        halo.push_back(halo.back() + halo_cells(e->g));
+        solid.push_back(solid.back() + e->n_solid);
+        b->max_entries += stencil_bound(e);
+    }
+    for (int ph = 0; ph < 2; ++ph) {
+        if (int rc = upload_vec(b, b->step_args[ph], st[ph], moved)) return rc;
+        if (int rc = upload_vec(b, b->ghost_args[ph], gh[ph], moved)) return rc;
+        if (int rc = upload_vec(b, b->solid_args[ph], so[ph], moved)) return rc;
+        if (int rc = upload_vec(b, b->source_args[ph], sr[ph], moved)) return rc;
+    }
+    if (int rc = upload_vec(b, b->tile_start, tiles, moved)) return rc;
+    if (int rc = upload_vec(b, b->halo_start, halo, moved)) return rc;
+    if (int rc = upload_vec(b, b->solid_start, solid, moved)) return rc;
+    b->total_tiles = tiles.back();
+    b->total_halo = halo.back();
+    b->total_solid = solid.back();
+    int occ = 0;
+    HIP_TRY(by_mode(b->mode, [&](auto L) {
+        occ = decltype(L)::step_ab_multi_occupancy(b->block);
+        return hipSuccess;
+    }));
+    const long long resident = (long long)std::max(1, occ) * std::max(1, b->n_cu);
+    b->sweep_grid = (unsigned)std::max(1LL, std::min(resident, b->total_tiles));
+    (void)n;
+    HIP_TRY(hipStreamSynchronize(b->stream));   // the host vectors above go out of scope
+    return NLOS_OK;
+}
+
+SourceMulti batch_source_multi(const nlos_batch *b, int phase) {
+    SourceMulti m{};
+    m.scenes = b->source_args[phase].as<SourceArgs>();
+    m.ctr = b->ctr.as<long long>();
+    m.ctr_base = b->ctr.as<long long>() + 1;
+    m.step_off = b->sched_off.as<long long>();
+    m.entry_scene = b->sched_scene.as<int>();
+    m.cells = b->sched_cells.as<long long>();
+    m.adds = b->sched_adds.as<double>();
+    return m;
+}
+
+// One step of every scene in the given phase.
+int batch_launch_step(nlos_batch *b, int phase) {
+    const int n = (int)b->scenes.size();
+    const SourceMulti m = batch_source_multi(b, phase);
+    HIP_TRY(by_mode(b->mode, [&](auto L) {
+        using LL = decltype(L);
+        hipError_t r = LL::source_multi(m, b->max_entries, b->stream);
+        if (r == hipSuccess)
+            r = LL::ghost_multi(b->ghost_args[phase].as<GhostArgs>(), b->halo_start.as<long long>(), n,
+                                b->total_halo, b->stream);
+        if (r == hipSuccess)
+            r = LL::solid_multi(b->solid_args[phase].as<SolidArgs>(), b->solid_start.as<long long>(), n,
+                                b->total_solid, b->stream);
+        if (r == hipSuccess)
+            r = LL::step_ab_multi(b->step_args[phase].as<StepArgs>(), b->tile_start.as<long long>(), n,
+                                  b->sweep_grid, b->block, b->stream);
+        return r;
+    }));
+    HIP_TRY(launch_count(b->ctr.as<long long>(), 1, b->stream));
+    return NLOS_OK;
+}
+
+// The source schedule of steps step .. step+m-1 of every scene, CSR over steps.
+int batch_schedule(nlos_batch *b, long long m, bool &moved) {
+    std::vector<long long> off(1, 0), cells;
+    std::vector<int> scene;
+    std::vector<double> adds;
+    for (long long s = 0; s < m; ++s) {
+        for (size_t i = 0; i < b->scenes.size(); ++i) {
+            nlos_engine *e = b->scenes[i];
+            e->sched_cells.clear();
+            e->sched_adds.clear();
+            schedule_step(e, (double)(b->step + s) * e->dt);
+            for (size_t k = 0; k < e->sched_cells.size(); ++k) {
+                cells.push_back(e->sched_cells[k]);
+                adds.push_back(e->sched_adds[k]);
+                scene.push_back((int)i);
+            }
+        }
+        off.push_back((long long)cells.size());
+    }
+    if (int rc = upload_vec(b, b->sched_off, off, moved)) return rc;
+    if (int rc = upload_vec(b, b->sched_scene, scene, moved)) return rc;
+    if (int rc = upload_vec(b, b->sched_cells, cells, moved)) return rc;
+    if (int rc = upload_vec(b, b->sched_adds, adds, moved)) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return NLOS_OK;
+}
+
+int batch_capture(nlos_batch *b) {
+    drop_graph(b);
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_launch_step(b, 0);
+    if (!rc) rc = batch_launch_step(b, 1);
+    hipGraph_t g = nullptr;
+    const hipError_t he = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (he != hipSuccess) return fail(NLOS_ERR_HIP, "graph capture: %s", hipGetErrorString(he));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return NLOS_OK;
+}
+
+}  // namespace
+
+NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **out) {
+    if (!cfgs || n <= 0 || !out) return fail(NLOS_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (cfgs[i].precision != cfgs[0].precision) return fail(NLOS_ERR_ARG, "scenes of a batch share one precision");
+        if (cfgs[i].stream_pattern == NLOS_PATTERN_INPLACE)
+            return fail(NLOS_ERR_UNSUPPORTED, "batched scenes use the two-array pattern");
+    }
+    nlos_batch *b = new nlos_batch();
+    auto bail = [&](int rc) { nlos_batch_destroy(b); return rc; };
+    b->mode = cfgs[0].precision;
+    if (const char *x = std::getenv("NLOS_BATCH_BLOCK")) parse_block(x, b->block);
+    if (const char *x = std::getenv("NLOS_BATCH_COLUMN")) b->col_planes = std::max(1, std::atoi(x));
+    if (hipGetDevice(&b->device) != hipSuccess ||
+        hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
+        return bail(fail(NLOS_ERR_HIP, "device query failed"));
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(NLOS_ERR_HIP, "stream creation failed"));
+    if (b->ctr.reserve(2 * sizeof(long long)) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "counter"));
+    for (int i = 0; i < n; ++i) {
+        nlos_config c = cfgs[i];
+        c.stream_pattern = NLOS_PATTERN_TWO_ARRAY;
+        nlos_engine *e = nullptr;
+        if (int rc = nlos_create(&c, &e)) {
+            const std::string msg = g_err;
+            return bail(fail(rc, "scene %d: %s", i, msg.c_str()));
+        }
+        e->batch = b;
+        b->scenes.push_back(e);
+    }
+    *out = b;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_destroy(nlos_batch *b) {
+    if (!b) return NLOS_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (nlos_engine *e : b->scenes) {
+        e->batch = nullptr;
+        nlos_destroy(e);
+    }
+    DevBuf *bufs[] = {&b->step_args[0], &b->step_args[1], &b->ghost_args[0], &b->ghost_args[1],
+                      &b->solid_args[0], &b->solid_args[1], &b->source_args[0], &b->source_args[1],
+                      &b->tile_start, &b->halo_start, &b->solid_start, &b->ctr, &b->sched_off,
+                      &b->sched_scene, &b->sched_cells, &b->sched_adds};
+    for (DevBuf *d : bufs) d->release();
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_size(const nlos_batch *b, int32_t *n) {
+    if (int rc = batch_ready(b)) return rc;
+    if (!n) return fail(NLOS_ERR_ARG, "null output");
+    *n = (int32_t)b->scenes.size();
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_scene(nlos_batch *b, int32_t i, nlos_engine **scene) {
+    if (int rc = batch_ready(b)) return rc;
+    if (!scene || i < 0 || i >= (int32_t)b->scenes.size()) return fail(NLOS_ERR_ARG, "scene index out of range");
+    *scene = b->scenes[i];
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result) {
+    if (int rc = batch_ready(b)) return rc;
+    if (reps < 1) return fail(NLOS_ERR_ARG, "reps must be positive");
+    const dim3 blocks[] = {dim3(64, 4, 1), dim3(128, 2, 1), dim3(256, 1, 1)};
+    const int cols[] = {1, 2, 4, 8, 16};
+    if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
+    if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
+    const dim3 keep_block = b->block;
+    const int keep_col = b->col_planes;
+    double best = 1e300;
+    dim3 best_block = keep_block;
+    int best_col = keep_col, tried = 0;
+    bool moved = false;
+    for (const dim3 &bl : blocks)
+        for (int col : cols) {
+            b->block = bl;
+            b->col_planes = col;
+            if (int rc = batch_prepare(b, moved)) return rc;
+            // the sweep alone, into the array the next step overwrites: no probe rows, no
+            // face records
+            std::vector<StepArgs> st;
+            for (nlos_engine *e : b->scenes) {
+                StepArgs a = step_args(e, 0, b->block, b->col_planes);
+                a.n_probe_cells = 0;
+                a.fb[0] = a.fb[1] = nullptr;
+                st.push_back(a);
+            }
+            if (int rc = upload_vec(b, b->step_args[0], st, moved)) return rc;
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            auto sweep = [&]() {
+                return by_mode(b->mode, [&](auto L) {
+                    return decltype(L)::step_ab_multi(b->step_args[0].as<StepArgs>(), b->tile_start.as<long long>(),
+                                                      (int)b->scenes.size(), b->sweep_grid, b->block, b->stream);
+                });
+            };
+            HIP_TRY(sweep());
+            HIP_TRY(hipEventRecord(b->ev0, b->stream));
+            for (int r = 0; r < reps; ++r) HIP_TRY(sweep());
+            HIP_TRY(hipEventRecord(b->ev1, b->stream));
+            HIP_TRY(hipEventSynchronize(b->ev1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            ++tried;
+            if (ms / reps < best) {
+                best = ms / reps;
+                best_block = bl;
+                best_col = col;
+            }
+        }
+    b->block = best_block;
+    b->col_planes = best_col;
+    drop_graph(b);
+    b->baked.clear();
+    if (result) {
+        result->block_x = (int32_t)best_block.x;
+        result->block_y = (int32_t)best_block.y;
+        result->col_planes = best_col;
+        result->candidates = tried;
+        result->best_ms = best;
+    }
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_step(nlos_batch *b, int64_t n) {
+    if (int rc = batch_ready(b)) return rc;
+    if (n < 0) return fail(NLOS_ERR_ARG, "negative step count");
+    if (n == 0) return NLOS_OK;
+    for (nlos_engine *e : b->scenes) {
+        if (e->step != b->step) return fail(NLOS_ERR_STATE, "scene stepped outside its batch");
+        if (int rc = ensure_hist(e, e->step + n)) return rc;
+    }
+    if (b->timing) {
+        if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
+        if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
+    }
+    const long long chunk = 2048;
+    for (long long base = 0; base < n; base += chunk) {
+        const long long m = std::min<long long>(chunk, n - base);
+        bool moved = false;
+        if (int rc = batch_prepare(b, moved)) return rc;
+        if (int rc = batch_schedule(b, m, moved)) return rc;
+        b->h_ctr[0] = b->step;
+        b->h_ctr[1] = b->step;
+        HIP_TRY(hipMemcpyAsync(b->ctr.p, b->h_ctr, sizeof b->h_ctr, hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        const std::vector<long long> shape = {
+            (long long)b->block.x, (long long)b->block.y, b->col_planes, (long long)b->sweep_grid, b->total_tiles,
+            b->total_halo, b->total_solid, b->max_entries};
+        if (moved || shape != b->baked) drop_graph(b);
+        if (m >= 2 && !b->exec) {
+            if (int rc = batch_capture(b)) return rc;
+            b->baked = shape;
+        }
+        if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->stream));
+        for (long long r = 0; r < m / 2; ++r) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+        if (m & 1) {
+            if (int rc = batch_launch_step(b, 0)) return rc;
+        }
+        if (b->timing) {
+            HIP_TRY(hipEventRecord(b->ev1, b->stream));
+            HIP_TRY(hipEventSynchronize(b->ev1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            b->timed_ms += ms;
+            b->timed_steps += m;
+        }
+        // phase bookkeeping: after m steps every scene is m steps on
+        for (nlos_engine *e : b->scenes) {
+            if (m & 1) {
+                e->cur ^= 1;
+                if (e->n_solid > 0) std::swap(e->s_a, e->s_b);
+            }
+            e->step += m;
+            e->fb_valid = true;
+        }
+        b->step += m;
+    }
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_synchronize(nlos_batch *b) {
+    if (int rc = batch_ready(b)) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (nlos_engine *e : b->scenes)
+        if (int rc = check_flag(e)) return rc;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_timing_enable(nlos_batch *b, int32_t on) {
+    if (int rc = batch_ready(b)) return rc;
+    b->timing = on != 0;
+    b->timed_ms = 0.0;
+    b->timed_steps = 0;
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_timing_get(nlos_batch *b, nlos_timing *t) {
+    if (int rc = batch_ready(b)) return rc;
+    if (!t) return fail(NLOS_ERR_ARG, "null timing");
+    t->launches = b->timed_steps;
+    t->total_ms = b->timed_ms;
+    t->avg_ms = b->timed_steps ? b->timed_ms / (double)b->timed_steps : 0.0;
     return NLOS_OK;
 }

@@ -1154,6 +1154,46 @@ hipError_t Launch<MODE>::fb_gather(const void *pop, const uint8_t *mask, const G
 }
 
 template <int MODE>
+hipError_t Launch<MODE>::step_ab_multi(const StepArgs *scenes, const long long *tile_start, int n, unsigned grid,
+                                       dim3 block, hipStream_t s) {
+    if (n <= 0 || grid == 0) return hipSuccess;
+    k_step_ab_multi<MODE><<<grid, block, 0, s>>>(scenes, tile_start, n);
+    return hipGetLastError();
+}
+
+template <int MODE>
+int Launch<MODE>::step_ab_multi_occupancy(dim3 block) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step_ab_multi<MODE>, (int)(block.x * block.y), 0) !=
+        hipSuccess)
+        return 0;
+    return n;
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::ghost_multi(const GhostArgs *scenes, const long long *start, int n, long long total,
+                                     hipStream_t s) {
+    if (n <= 0 || total <= 0) return hipSuccess;
+    k_ghost_multi<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(scenes, start, n);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::solid_multi(const SolidArgs *scenes, const long long *start, int n, long long total,
+                                     hipStream_t s) {
+    if (n <= 0 || total <= 0) return hipSuccess;
+    k_solid_multi<MODE><<<blocks_for(total, 256), 256, 0, s>>>(scenes, start, n);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::source_multi(const SourceMulti &m, long long max_entries, hipStream_t s) {
+    if (max_entries <= 0) return hipSuccess;
+    k_source_multi<MODE, LAY_AB><<<blocks_for(max_entries * Q, 256), 256, 0, s>>>(m);
+    return hipGetLastError();
+}
+
+template <int MODE>
 hipError_t Launch<MODE>::gather_f(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
                                   float inv_scale, double *out, hipStream_t s) {
     const long long total = (long long)g.nx * g.ny * g.nz;

@@ -69,6 +69,11 @@ class Timing(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("avg_ms", C.c_double)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("block_x", C.c_int32), ("block_y", C.c_int32), ("col_planes", C.c_int32),
+                ("candidates", C.c_int32), ("best_ms", C.c_double)]
+
+
 # Every C-ABI entry point of include/nlos_engine.h (plus the test hooks of
 # include/nlos_engine_test.h) with its ctypes signature.
 _P, _E = C.c_void_p, C.c_void_p
@@ -102,6 +107,15 @@ SIGNATURES = {
     "nlos_timing_enable": (C.c_int, [_E, C.c_int32]),
     "nlos_timing_reset": (C.c_int, [_E]),
     "nlos_timing_get": (C.c_int, [_E, C.POINTER(Timing)]),
+    "nlos_batch_create": (C.c_int, [C.POINTER(Config), C.c_int32, C.POINTER(C.c_void_p)]),
+    "nlos_batch_destroy": (C.c_int, [_P]),
+    "nlos_batch_size": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "nlos_batch_scene": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_void_p)]),
+    "nlos_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Tune)]),
+    "nlos_batch_step": (C.c_int, [_P, C.c_int64]),
+    "nlos_batch_synchronize": (C.c_int, [_P]),
+    "nlos_batch_timing_enable": (C.c_int, [_P, C.c_int32]),
+    "nlos_batch_timing_get": (C.c_int, [_P, C.POINTER(Timing)]),
     "nlos_debug_read_f": (C.c_int, [_E, _P]),
     "nlos_debug_poke": (C.c_int, [_E, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
     "nlos_debug_copy_kernel": (C.c_int, [C.c_size_t, C.c_int32, C.POINTER(C.c_double)]),
@@ -169,10 +183,14 @@ def _ptr(a: np.ndarray) -> C.c_void_p:
 class Engine:
     """One simulation domain on the current GPU (the reference's `LBMSolver`)."""
 
-    def __init__(self, cfg: Config):
+    def __init__(self, cfg: Config, _handle: Optional[C.c_void_p] = None):
         self.lib = load_library()
-        h = C.c_void_p()
-        _check(self.lib.nlos_create(C.byref(cfg), C.byref(h)))
+        self.owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            _check(self.lib.nlos_create(C.byref(cfg), C.byref(h)))
+        else:
+            h = _handle
         self.h = h
         self.cfg = cfg
         self._samples = None
@@ -181,7 +199,8 @@ class Engine:
     # -- lifetime --------------------------------------------------------------------
     def close(self) -> None:
         if getattr(self, "h", None):
-            self.lib.nlos_destroy(self.h)
+            if self.owned:
+                self.lib.nlos_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -386,3 +405,64 @@ def reference05_source() -> Source:
     """Static Gaussian pulse of 05:1156-1164."""
     return default_source(sx=912.0, sy=66.6, sz=86.0, ex=912.0, ey=66.6, ez=86.0,
                           speed=0.0, amp=10.0, freq=0.0)
+
+
+class Batch:
+    """Independent scenes advanced together (multi-scene launches, replayed HIP graph).
+    scene(i) returns an Engine view of scene i for set-up and read-back; stepping goes
+    through Batch.step."""
+
+    def __init__(self, cfgs: Sequence[Config]):
+        self.lib = load_library()
+        arr = (Config * len(cfgs))(*cfgs)
+        h = C.c_void_p()
+        _check(self.lib.nlos_batch_create(arr, len(cfgs), C.byref(h)))
+        self.h = h
+        self.scenes = []
+        for i, cfg in enumerate(cfgs):
+            sh = C.c_void_p()
+            _check(self.lib.nlos_batch_scene(self.h, i, C.byref(sh)))
+            self.scenes.append(Engine(cfg, _handle=sh))
+
+    def __len__(self) -> int:
+        return len(self.scenes)
+
+    def scene(self, i: int) -> "Engine":
+        return self.scenes[i]
+
+    def autotune(self, reps: int = 3) -> Tune:
+        t = Tune()
+        _check(self.lib.nlos_batch_autotune(self.h, reps, C.byref(t)))
+        return t
+
+    def step(self, n: int) -> None:
+        _check(self.lib.nlos_batch_step(self.h, int(n)))
+
+    def synchronize(self) -> None:
+        _check(self.lib.nlos_batch_synchronize(self.h))
+
+    def timing(self, on: Optional[bool] = None) -> Timing:
+        if on is not None:
+            _check(self.lib.nlos_batch_timing_enable(self.h, 1 if on else 0))
+        t = Timing()
+        _check(self.lib.nlos_batch_timing_get(self.h, C.byref(t)))
+        return t
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            for e in self.scenes:
+                e.h = None
+            self.lib.nlos_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

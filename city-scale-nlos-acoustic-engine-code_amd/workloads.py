@@ -175,3 +175,54 @@ class ScatterWorkload:
                 continue
             pts.append(tuple(p))
         return np.array(pts)
+
+
+@dataclass
+class BatchScene:
+    cfg: nlos.Config
+    source: nlos.Source
+    probes: np.ndarray
+    dims: Tuple[int, int, int]
+
+
+@dataclass
+class BatchWorkload:
+    """Independent small scenes: dims uniform in [64,192] x [64,192] x [64,96] cells at
+    dx = 1 m, a point source (Ricker, seeded centre frequency 40-120 Hz) at a seeded height
+    of 2-20 m over a partially reflecting ground (bounce-back weight uniform 0.6-1.0), half
+    of the sources moving at uniform 1-10 m/s in a seeded horizontal direction, 4 probes at
+    1.5 m height, 16-cell sponge on the five open faces, no buildings."""
+    n_scenes: int = 256
+    seed: int = 0
+    precision: str = "f32"
+    tau: float = 0.5 + 1e-2
+    lo: Tuple[int, int, int] = (64, 64, 64)
+    hi: Tuple[int, int, int] = (192, 192, 96)
+    sponge_cells: int = 16
+
+    def scenes(self) -> List[BatchScene]:
+        rng = np.random.Generator(np.random.PCG64(self.seed))
+        out = []
+        for s in range(self.n_scenes):
+            nx, ny, nz = (int(rng.integers(self.lo[a], self.hi[a] + 1)) for a in range(3))
+            cfg = nlos.default_config(x_min=0.0, x_max=float(nx), y_min=0.0, y_max=float(ny),
+                                      z_min=0.0, z_max=float(nz), dx=1.0, c_phys=343.0, tau=self.tau,
+                                      sponge_width=float(self.sponge_cells), sponge_strength=0.1,
+                                      sky_z=1e9, wall_reflect=float(rng.uniform(0.6, 1.0)),
+                                      precision=self.precision, world_offset=(0.0, 0.0, 0.0))
+            h = float(rng.uniform(2.0, 20.0))
+            x0 = float(rng.uniform(0.35, 0.65)) * nx
+            y0 = float(rng.uniform(0.35, 0.65)) * ny
+            moving = s % 2 == 1
+            speed = float(rng.uniform(1.0, 10.0)) if moving else 0.0
+            ang = float(rng.uniform(0.0, 2.0 * math.pi))
+            x1, y1 = (x0 + 1000.0 * math.cos(ang), y0 + 1000.0 * math.sin(ang)) if moving else (x0, y0)
+            f0 = float(rng.uniform(40.0, 120.0))
+            src = nlos.default_source(sx=x0, sy=y0, sz=h, ex=x1, ey=y1, ez=h, speed=speed, amp=1e-2,
+                                      freq=f0, t0=1.5 / f0, waveform=nlos.WAVE_RICKER)
+            pr = []
+            for _ in range(4):
+                pr.append((float(rng.uniform(self.sponge_cells, nx - self.sponge_cells)),
+                           float(rng.uniform(self.sponge_cells, ny - self.sponge_cells)), 1.5))
+            out.append(BatchScene(cfg, src, np.array(pr), (nx, ny, nz)))
+        return out

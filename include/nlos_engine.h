@@ -210,6 +210,43 @@ int nlos_timing_enable(nlos_engine *e, int32_t on);
 int nlos_timing_reset(nlos_engine *e);
 int nlos_timing_get(nlos_engine *e, nlos_timing *t);
 
+/* ---- batched scenes ------------------------------------------------------------------
+ * No counterpart in the reference, which steps one domain per process (05:1168-1170).  A
+ * batch owns n independent scenes, each an nlos_engine of its own configuration (same
+ * precision for all; two-array stream pattern) that the caller sets up through the
+ * ordinary calls (nlos_set_mask, nlos_set_source, nlos_set_probes, nlos_set_density ...)
+ * and reads back through them (nlos_read_probes ...).  nlos_batch_step advances every scene
+ * together: per step one source, one ghost, one boundary-solid and one persistent sweep
+ * launch cover all scenes, and pairs of steps replay one captured HIP graph.  A scene of a
+ * batch cannot be stepped, restored or destroyed on its own (NLOS_ERR_STATE).  Each scene's
+ * results are bit-identical to the same scene stepped alone. */
+typedef struct nlos_batch nlos_batch;
+
+typedef struct nlos_tune {
+    int32_t block_x, block_y;   /* workgroup shape of the sweep (block_x * block_y = 256) */
+    int32_t col_planes;         /* planes walked per workgroup */
+    int32_t candidates;         /* shapes timed */
+    double best_ms;             /* sweep time of the chosen shape, all scenes, one step */
+} nlos_tune;
+
+int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **out);
+int nlos_batch_destroy(nlos_batch *b);
+int nlos_batch_size(const nlos_batch *b, int32_t *n);
+/* The engine handle of scene i (owned by the batch). */
+int nlos_batch_scene(nlos_batch *b, int32_t i, nlos_engine **scene);
+/* Times the sweep of all scenes for each candidate workgroup shape and column length (reps
+ * launches each, into the array the next step overwrites, so the state is untouched) and
+ * keeps the fastest.  result may be NULL. */
+int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result);
+/* n steps of every scene (each: apply_source(s*dt), exchange_halos, step_collision, probe
+ * record, as nlos_step). */
+int nlos_batch_step(nlos_batch *b, int64_t n);
+int nlos_batch_synchronize(nlos_batch *b);
+/* When on, each nlos_batch_step call is bracketed by HIP events on the batch stream;
+ * nlos_timing.launches counts steps and avg_ms is the time per step of all scenes. */
+int nlos_batch_timing_enable(nlos_batch *b, int32_t on);
+int nlos_batch_timing_get(nlos_batch *b, nlos_timing *t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,99 @@
+"""Batched scenes: every scene of a batch gives the same bits as the same scene stepped
+alone (the batch changes only how launches are grouped), over runs that mix replayed
+two-step graphs with a single trailing step, across consecutive step calls, and after the
+workgroup-shape autotuner ran."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import nlos_engine as nlos
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _defs(precision):
+    """Uneven scenes: buildings crossing open faces (tracked boundary solids), a partial
+    last row segment, rows shorter than 19 cells, reflecting walls with trilinear Ricker
+    injection, and a source-free pulse scene."""
+    out = []
+    cfg, solid, src, probes = scenes.box_scene(precision)
+    out.append((cfg, solid, src, probes, None))
+    cfg, solid, src, probes = scenes.box_scene(precision, nx=100, ny=12, nz=37, seed=5)
+    out.append((cfg, solid, src, probes, None))
+    cfg, solid, src, probes = scenes.reflect_scene(precision)
+    out.append((cfg, solid, src, probes, None))
+    cfg, solid, src, probes = scenes.box_scene(precision, nx=12, ny=20, nz=14, seed=7)
+    out.append((cfg, solid, src, probes, None))
+    cfg, solid, rho, probes = scenes.pulse_scene(precision)
+    out.append((cfg, solid, None, probes, rho))
+    for d in out:
+        d[0].stream_pattern = nlos.PATTERN_TWO_ARRAY
+    return out
+
+
+def _setup(e, d):
+    cfg, solid, src, probes, rho = d
+    e.set_mask(solid)
+    if rho is not None:
+        e.set_density(rho)
+    if src is not None:
+        e.set_source(src)
+    e.set_probes(probes)
+
+
+def _single(d, steps):
+    e = nlos.Engine(d[0])
+    _setup(e, d)
+    e.step(steps)
+    e.synchronize()
+    res = (e.read_probes(), e.debug_read_f())
+    e.close()
+    return res
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "f16s"])
+def test_batch_equals_scenes_stepped_alone(gpu, precision):
+    print(f"[batch vs single] precision={precision}", flush=True)
+    defs = _defs(precision)
+    b = nlos.Batch([d[0] for d in defs])
+    for i, d in enumerate(defs):
+        _setup(b.scene(i), d)
+    b.step(7)       # three graph replays of two steps and one direct step
+    b.step(10)      # the same graph again, from the other phase
+    b.synchronize()
+    for i, d in enumerate(defs):
+        want_p, want_f = _single(d, 17)
+        got_p, got_f = b.scene(i).read_probes(), b.scene(i).debug_read_f()
+        assert got_p.shape == want_p.shape
+        assert np.array_equal(got_p, want_p), (i, np.abs(got_p - want_p).max())
+        assert np.array_equal(got_f, want_f, equal_nan=True), i
+    b.close()
+
+
+def test_batch_autotune_keeps_state_and_results(gpu):
+    defs = _defs("f32")
+    b = nlos.Batch([d[0] for d in defs])
+    for i, d in enumerate(defs):
+        _setup(b.scene(i), d)
+    b.step(3)
+    t = b.autotune(reps=1)
+    assert t.candidates >= 3 and t.block_x * t.block_y == 256 and t.best_ms > 0.0
+    b.step(6)
+    b.synchronize()
+    for i, d in enumerate(defs):
+        want_p, want_f = _single(d, 9)
+        assert np.array_equal(b.scene(i).read_probes(), want_p), i
+        assert np.array_equal(b.scene(i).debug_read_f(), want_f, equal_nan=True), i
+    b.close()
+
+
+def test_batch_scene_cannot_step_alone(gpu):
+    defs = _defs("f32")[:2]
+    b = nlos.Batch([d[0] for d in defs])
+    with pytest.raises(nlos.EngineError):
+        b.scene(0).step(1)
+    b.step(2)
+    assert b.scene(1).info().step == 2
+    b.close()
