@@ -13,6 +13,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1271,8 +1272,8 @@ NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) 
 // Batched scenes.  Every scene is an ordinary engine (two-array pattern); the batch owns
 // them and advances them together with multi-scene launches: per step one source launch
 // (entries of all scenes, indexed by a device step counter), one ghost and one boundary
-// solid launch over the concatenated halo / solid ranges, one persistent sweep over the
-// concatenated tile ranges, and a counter increment.  Kernel arguments of every scene for
+// solid launch over the concatenated halo / solid ranges, one sweep whose workgroups each
+// walk a run of consecutive tiles of one scene, and a counter increment.  Kernel arguments of every scene for
 // the current state (phase 0) and the state one step later (phase 1) sit in device arrays,
 // so a captured graph of two steps is replayed for every pair of steps.
 struct nlos_batch {
@@ -1283,10 +1284,10 @@ struct nlos_batch {
     hipStream_t stream = nullptr;
     dim3 block{256, 1, 1};
     int col_planes = 4;
+    int per_group = 4;                    // tiles walked by one sweep workgroup
     DevBuf step_args[2], ghost_args[2], solid_args[2], source_args[2];
-    DevBuf tile_start, halo_start, solid_start, ctr;
-    long long total_tiles = 0, total_halo = 0, total_solid = 0, max_entries = 0;
-    unsigned sweep_grid = 0;
+    DevBuf tile_table, halo_blocks, solid_blocks, ctr;
+    long long total_tiles = 0, total_halo = 0, total_solid = 0, max_entries = 0;   // table sizes
     DevBuf sched_off, sched_scene, sched_cells, sched_adds;
     long long h_ctr[2] = {0, 0};          // host staging of the counter and its run base
     hipGraph_t graph = nullptr;
@@ -1341,7 +1342,7 @@ int batch_prepare(nlos_batch *b, bool &moved) {
     std::vector<GhostArgs> gh[2];
     std::vector<SolidArgs> so[2];
     std::vector<SourceArgs> sr[2];
-    std::vector<long long> tiles(1, 0), halo(1, 0), solid(1, 0);
+    std::vector<int2> tiles, halo, solid;   // work tables (scene, tile / first item)
     b->max_entries = 0;
     for (nlos_engine *e : b->scenes) {
         HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1357,9 +1358,10 @@ int batch_prepare(nlos_batch *b, bool &moved) {
             so[ph].push_back(solid_args(e, ph));
             sr[ph].push_back(source_args(e, ph));
         }
-        tiles.push_back(tiles.back() + st[0].back().nblocks);
-        halo.push_back(halo.back() + halo_cells(e->g));
-        solid.push_back(solid.back() + e->n_solid);
+        const int scene = (int)st[0].size() - 1;
+        for (int t = 0; t < st[0].back().nblocks; t += b->per_group) tiles.push_back(make_int2(scene, t));
+        for (long long h = 0; h < halo_cells(e->g); h += 256) halo.push_back(make_int2(scene, (int)h));
+        for (int q = 0; q < e->n_solid; q += 256) solid.push_back(make_int2(scene, q));
         b->max_entries += stencil_bound(e);
     }
     for (int ph = 0; ph < 2; ++ph) {
@@ -1368,19 +1370,13 @@ int batch_prepare(nlos_batch *b, bool &moved) {
         if (int rc = upload_vec(b, b->solid_args[ph], so[ph], moved)) return rc;
         if (int rc = upload_vec(b, b->source_args[ph], sr[ph], moved)) return rc;
     }
-    if (int rc = upload_vec(b, b->tile_start, tiles, moved)) return rc;
-    if (int rc = upload_vec(b, b->halo_start, halo, moved)) return rc;
-    if (int rc = upload_vec(b, b->solid_start, solid, moved)) return rc;
-    b->total_tiles = tiles.back();
-    b->total_halo = halo.back();
-    b->total_solid = solid.back();
-    int occ = 0;
-    HIP_TRY(by_mode(b->mode, [&](auto L) {
-        occ = decltype(L)::step_ab_multi_occupancy(b->block);
-        return hipSuccess;
-    }));
-    const long long resident = (long long)std::max(1, occ) * std::max(1, b->n_cu);
-    b->sweep_grid = (unsigned)std::max(1LL, std::min(resident, b->total_tiles));
+    if (tiles.size() > (size_t)INT32_MAX) return fail(NLOS_ERR_UNSUPPORTED, "too many tiles in one batch");
+    if (int rc = upload_vec(b, b->tile_table, tiles, moved)) return rc;
+    if (int rc = upload_vec(b, b->halo_blocks, halo, moved)) return rc;
+    if (int rc = upload_vec(b, b->solid_blocks, solid, moved)) return rc;
+    b->total_tiles = (long long)tiles.size();
+    b->total_halo = (long long)halo.size();
+    b->total_solid = (long long)solid.size();
     (void)n;
     HIP_TRY(hipStreamSynchronize(b->stream));   // the host vectors above go out of scope
     return NLOS_OK;
@@ -1406,14 +1402,14 @@ int batch_launch_step(nlos_batch *b, int phase) {
         using LL = decltype(L);
         hipError_t r = LL::source_multi(m, b->max_entries, b->stream);
         if (r == hipSuccess)
-            r = LL::ghost_multi(b->ghost_args[phase].as<GhostArgs>(), b->halo_start.as<long long>(), n,
-                                b->total_halo, b->stream);
+            r = LL::ghost_multi(b->ghost_args[phase].as<GhostArgs>(), b->halo_blocks.as<int2>(),
+                                (int)b->total_halo, b->stream);
         if (r == hipSuccess)
-            r = LL::solid_multi(b->solid_args[phase].as<SolidArgs>(), b->solid_start.as<long long>(), n,
-                                b->total_solid, b->stream);
+            r = LL::solid_multi(b->solid_args[phase].as<SolidArgs>(), b->solid_blocks.as<int2>(),
+                                (int)b->total_solid, b->stream);
         if (r == hipSuccess)
-            r = LL::step_ab_multi(b->step_args[phase].as<StepArgs>(), b->tile_start.as<long long>(), n,
-                                  b->sweep_grid, b->block, b->stream);
+            r = LL::step_ab_multi(b->step_args[phase].as<StepArgs>(), b->tile_table.as<int2>(),
+                                  (int)b->total_tiles, b->per_group, b->block, b->stream);
         return r;
     }));
     HIP_TRY(launch_count(b->ctr.as<long long>(), 1, b->stream));
@@ -1422,16 +1418,35 @@ int batch_launch_step(nlos_batch *b, int phase) {
 
 // The source schedule of steps step .. step+m-1 of every scene, CSR over steps.
 int batch_schedule(nlos_batch *b, long long m, bool &moved) {
+    // Each scene's entries for all m steps (host threads over scenes; every scene uses
+    // only its own engine's buffers), then merged step-major.
+    const size_t n = b->scenes.size();
+    std::vector<std::vector<long long>> s_off(n);
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nt = std::min<size_t>(hw, n);
+    auto work = [&](size_t t0) {
+        for (size_t i = t0; i < n; i += nt) {
+            nlos_engine *e = b->scenes[i];
+            e->sched_cells.clear();
+            e->sched_adds.clear();
+            s_off[i].assign(1, 0);
+            for (long long s = 0; s < m; ++s) {
+                schedule_step(e, (double)(b->step + s) * e->dt);
+                s_off[i].push_back((long long)e->sched_cells.size());
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
     std::vector<long long> off(1, 0), cells;
     std::vector<int> scene;
     std::vector<double> adds;
     for (long long s = 0; s < m; ++s) {
-        for (size_t i = 0; i < b->scenes.size(); ++i) {
-            nlos_engine *e = b->scenes[i];
-            e->sched_cells.clear();
-            e->sched_adds.clear();
-            schedule_step(e, (double)(b->step + s) * e->dt);
-            for (size_t k = 0; k < e->sched_cells.size(); ++k) {
+        for (size_t i = 0; i < n; ++i) {
+            const nlos_engine *e = b->scenes[i];
+            for (long long k = s_off[i][s]; k < s_off[i][s + 1]; ++k) {
                 cells.push_back(e->sched_cells[k]);
                 adds.push_back(e->sched_adds[k]);
                 scene.push_back((int)i);
@@ -1479,6 +1494,7 @@ NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **
     b->mode = cfgs[0].precision;
     if (const char *x = std::getenv("NLOS_BATCH_BLOCK")) parse_block(x, b->block);
     if (const char *x = std::getenv("NLOS_BATCH_COLUMN")) b->col_planes = std::max(1, std::atoi(x));
+    if (const char *x = std::getenv("NLOS_BATCH_RUN")) b->per_group = std::max(1, std::atoi(x));
     if (hipGetDevice(&b->device) != hipSuccess ||
         hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
         return bail(fail(NLOS_ERR_HIP, "device query failed"));
@@ -1510,7 +1526,7 @@ NLOS_API int nlos_batch_destroy(nlos_batch *b) {
     }
     DevBuf *bufs[] = {&b->step_args[0], &b->step_args[1], &b->ghost_args[0], &b->ghost_args[1],
                       &b->solid_args[0], &b->solid_args[1], &b->source_args[0], &b->source_args[1],
-                      &b->tile_start, &b->halo_start, &b->solid_start, &b->ctr, &b->sched_off,
+                      &b->tile_table, &b->halo_blocks, &b->solid_blocks, &b->ctr, &b->sched_off,
                       &b->sched_scene, &b->sched_cells, &b->sched_adds};
     for (DevBuf *d : bufs) d->release();
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -1538,7 +1554,8 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
     if (int rc = batch_ready(b)) return rc;
     if (reps < 1) return fail(NLOS_ERR_ARG, "reps must be positive");
     const dim3 blocks[] = {dim3(64, 4, 1), dim3(128, 2, 1), dim3(256, 1, 1)};
-    const int cols[] = {1, 2, 4, 8, 16};
+    const int cols[] = {1, 4, 16};
+    const int runs[] = {1, 4, 16};
     if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
     if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
     const dim3 keep_block = b->block;
@@ -1546,11 +1563,14 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
     double best = 1e300;
     dim3 best_block = keep_block;
     int best_col = keep_col, tried = 0;
+    int best_run = b->per_group;
     bool moved = false;
+    for (int run : runs)
     for (const dim3 &bl : blocks)
         for (int col : cols) {
             b->block = bl;
             b->col_planes = col;
+            b->per_group = run;
             if (int rc = batch_prepare(b, moved)) return rc;
             // the sweep alone, into the array the next step overwrites: no probe rows, no
             // face records
@@ -1565,8 +1585,8 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
             HIP_TRY(hipStreamSynchronize(b->stream));
             auto sweep = [&]() {
                 return by_mode(b->mode, [&](auto L) {
-                    return decltype(L)::step_ab_multi(b->step_args[0].as<StepArgs>(), b->tile_start.as<long long>(),
-                                                      (int)b->scenes.size(), b->sweep_grid, b->block, b->stream);
+                    return decltype(L)::step_ab_multi(b->step_args[0].as<StepArgs>(), b->tile_table.as<int2>(),
+                                                      (int)b->total_tiles, b->per_group, b->block, b->stream);
                 });
             };
             HIP_TRY(sweep());
@@ -1581,17 +1601,21 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
                 best = ms / reps;
                 best_block = bl;
                 best_col = col;
+                best_run = b->per_group;
             }
         }
     b->block = best_block;
     b->col_planes = best_col;
+    b->per_group = best_run;
     drop_graph(b);
     b->baked.clear();
     if (result) {
         result->block_x = (int32_t)best_block.x;
         result->block_y = (int32_t)best_block.y;
         result->col_planes = best_col;
+        result->tiles_per_group = best_run;
         result->candidates = tried;
+        result->reserved = 0;
         result->best_ms = best;
     }
     return NLOS_OK;
@@ -1620,7 +1644,7 @@ NLOS_API int nlos_batch_step(nlos_batch *b, int64_t n) {
         HIP_TRY(hipMemcpyAsync(b->ctr.p, b->h_ctr, sizeof b->h_ctr, hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         const std::vector<long long> shape = {
-            (long long)b->block.x, (long long)b->block.y, b->col_planes, (long long)b->sweep_grid, b->total_tiles,
+            (long long)b->block.x, (long long)b->block.y, b->col_planes, (long long)b->per_group, b->total_tiles,
             b->total_halo, b->total_solid, b->max_entries};
         if (moved || shape != b->baked) drop_graph(b);
         if (m >= 2 && !b->exec) {
@@ -1665,8 +1689,10 @@ NLOS_API int nlos_batch_synchronize(nlos_batch *b) {
 NLOS_API int nlos_batch_timing_enable(nlos_batch *b, int32_t on) {
     if (int rc = batch_ready(b)) return rc;
     b->timing = on != 0;
-    b->timed_ms = 0.0;
-    b->timed_steps = 0;
+    if (b->timing) {
+        b->timed_ms = 0.0;
+        b->timed_steps = 0;
+    }
     return NLOS_OK;
 }
 

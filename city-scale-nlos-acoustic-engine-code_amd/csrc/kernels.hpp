@@ -656,15 +656,18 @@ __global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
     step_ab_body<MODE, CAPTURE>(a, b);
 }
 
-// Persistent sweep over the tiles of several scenes: each workgroup walks the global tile
-// range with stride gridDim.x and finds its scene by the tile prefix.
+// Sweep over the tiles of several scenes: workgroup g walks the run of `per_group`
+// consecutive tiles of one scene that runs[g] = (scene, first tile) names, so that the
+// scene's arguments are loaded once per run.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_step_ab_multi(const StepArgs *scenes, const long long *start, int n) {
-    const long long total = start[n];
-    for (long long b = blockIdx.x; b < total; b += gridDim.x) {
-        const int s = scene_of(start, n, b);
-        step_ab_body<MODE, false>(scenes[s], (int)(b - start[s]));
-    }
+__global__ __launch_bounds__(256) void k_step_ab_multi(const StepArgs *__restrict__ scenes,
+                                                       const int2 *__restrict__ runs, int per_group) {
+    const int2 r = runs[blockIdx.x];
+    // a private copy: read through the pointer, every field would be reloaded after each
+    // store of the sweep (the compiler cannot rule out aliasing)
+    const StepArgs a = scenes[r.x];
+    const int end = min(r.y + per_group, a.nblocks);
+    for (int b = r.y; b < end; ++b) step_ab_body<MODE, false>(a, b);
 }
 
 // Wall copies of LAY_P (see Layout): for every fluid cell c next to a wall and every q
@@ -866,12 +869,15 @@ __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
     ghost_body<MODE, LAY>(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// Several scenes in one launch: workgroup g serves scene blocks[g].x, items
+// blocks[g].y .. blocks[g].y + 255 of it (each scene's range starts on a new workgroup, so
+// the scene is uniform over the workgroup and its arguments are scalar loads).
 template <int MODE, int LAY>
-__global__ __launch_bounds__(256) void k_ghost_multi(const GhostArgs *scenes, const long long *start, int n) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= start[n]) return;
-    const int s = scene_of(start, n, t);
-    ghost_body<MODE, LAY>(scenes[s], t - start[s]);
+__global__ __launch_bounds__(256) void k_ghost_multi(const GhostArgs *__restrict__ scenes,
+                                                     const int2 *__restrict__ blocks) {
+    const int2 sb = blocks[blockIdx.x];
+    const GhostArgs a = scenes[sb.x];
+    ghost_body<MODE, LAY>(a, (long long)sb.y + threadIdx.x);
 }
 
 // ---- solid cells on open faces (05:917-938 for a SOLID dst) -------------------------------
@@ -906,11 +912,11 @@ __global__ __launch_bounds__(256) void k_solid(SolidArgs a) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_solid_multi(const SolidArgs *scenes, const long long *start, int n) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= start[n]) return;
-    const int s = scene_of(start, n, t);
-    solid_body<MODE>(scenes[s], (int)(t - start[s]));
+__global__ __launch_bounds__(256) void k_solid_multi(const SolidArgs *__restrict__ scenes,
+                                                     const int2 *__restrict__ blocks) {
+    const int2 sb = blocks[blockIdx.x];
+    const SolidArgs a = scenes[sb.x];
+    solid_body<MODE>(a, sb.y + (int)threadIdx.x);
 }
 
 // ---- source injection (05:660-665) ---------------------------------------------------------
@@ -1154,35 +1160,24 @@ hipError_t Launch<MODE>::fb_gather(const void *pop, const uint8_t *mask, const G
 }
 
 template <int MODE>
-hipError_t Launch<MODE>::step_ab_multi(const StepArgs *scenes, const long long *tile_start, int n, unsigned grid,
+hipError_t Launch<MODE>::step_ab_multi(const StepArgs *scenes, const int2 *runs, int n_runs, int per_group,
                                        dim3 block, hipStream_t s) {
-    if (n <= 0 || grid == 0) return hipSuccess;
-    k_step_ab_multi<MODE><<<grid, block, 0, s>>>(scenes, tile_start, n);
+    if (n_runs <= 0) return hipSuccess;
+    k_step_ab_multi<MODE><<<n_runs, block, 0, s>>>(scenes, runs, per_group);
     return hipGetLastError();
 }
 
 template <int MODE>
-int Launch<MODE>::step_ab_multi_occupancy(dim3 block) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step_ab_multi<MODE>, (int)(block.x * block.y), 0) !=
-        hipSuccess)
-        return 0;
-    return n;
-}
-
-template <int MODE>
-hipError_t Launch<MODE>::ghost_multi(const GhostArgs *scenes, const long long *start, int n, long long total,
-                                     hipStream_t s) {
-    if (n <= 0 || total <= 0) return hipSuccess;
-    k_ghost_multi<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(scenes, start, n);
+hipError_t Launch<MODE>::ghost_multi(const GhostArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s) {
+    if (n_blocks <= 0) return hipSuccess;
+    k_ghost_multi<MODE, LAY_AB><<<n_blocks, 256, 0, s>>>(scenes, blocks);
     return hipGetLastError();
 }
 
 template <int MODE>
-hipError_t Launch<MODE>::solid_multi(const SolidArgs *scenes, const long long *start, int n, long long total,
-                                     hipStream_t s) {
-    if (n <= 0 || total <= 0) return hipSuccess;
-    k_solid_multi<MODE><<<blocks_for(total, 256), 256, 0, s>>>(scenes, start, n);
+hipError_t Launch<MODE>::solid_multi(const SolidArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s) {
+    if (n_blocks <= 0) return hipSuccess;
+    k_solid_multi<MODE><<<n_blocks, 256, 0, s>>>(scenes, blocks);
     return hipGetLastError();
 }
 

@@ -34,17 +34,14 @@ struct Launch {
     // Fill the x-face buffers (StepArgs::fb) from the lattice in layout lay.
     static hipError_t fb_gather(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
                                 void *fb0, void *fb1, float scale, float inv_scale, hipStream_t s);
-    // Several scenes of the two-array layout in one launch each (argument arrays and the
-    // prefix arrays start[0..n] of tiles / halo cells / tracked solids live on the device).
-    // The sweep is persistent: `grid` workgroups walk all scenes' tiles.
-    static hipError_t step_ab_multi(const StepArgs *scenes, const long long *tile_start, int n, unsigned grid,
+    // Several scenes of the two-array layout in one launch each.  Argument arrays and the
+    // work tables live on the device: runs[g] = (scene, first tile) of the `per_group`
+    // consecutive tiles that sweep workgroup g walks; blocks[g] = (scene, first item) for
+    // the 256-thread workgroups of the ghost and boundary-solid launches.
+    static hipError_t step_ab_multi(const StepArgs *scenes, const int2 *runs, int n_runs, int per_group,
                                     dim3 block, hipStream_t s);
-    static hipError_t ghost_multi(const GhostArgs *scenes, const long long *start, int n, long long total,
-                                  hipStream_t s);
-    static hipError_t solid_multi(const SolidArgs *scenes, const long long *start, int n, long long total,
-                                  hipStream_t s);
-    // Resident workgroups of the persistent sweep per CU for a workgroup shape.
-    static int step_ab_multi_occupancy(dim3 block);
+    static hipError_t ghost_multi(const GhostArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
+    static hipError_t solid_multi(const SolidArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
     // Source entries of the step named by the device counter; at most max_entries per step.
     static hipError_t source_multi(const SourceMulti &m, long long max_entries, hipStream_t s);
     // Gather f* of every interior cell into AoS FP64 full populations [cell][q] over the

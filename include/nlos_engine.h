@@ -216,8 +216,8 @@ int nlos_timing_get(nlos_engine *e, nlos_timing *t);
  * precision for all; two-array stream pattern) that the caller sets up through the
  * ordinary calls (nlos_set_mask, nlos_set_source, nlos_set_probes, nlos_set_density ...)
  * and reads back through them (nlos_read_probes ...).  nlos_batch_step advances every scene
- * together: per step one source, one ghost, one boundary-solid and one persistent sweep
- * launch cover all scenes, and pairs of steps replay one captured HIP graph.  A scene of a
+ * together: per step one source, one ghost, one boundary-solid and one sweep launch
+ * cover all scenes, and pairs of steps replay one captured HIP graph.  A scene of a
  * batch cannot be stepped, restored or destroyed on its own (NLOS_ERR_STATE).  Each scene's
  * results are bit-identical to the same scene stepped alone. */
 typedef struct nlos_batch nlos_batch;
@@ -225,7 +225,9 @@ typedef struct nlos_batch nlos_batch;
 typedef struct nlos_tune {
     int32_t block_x, block_y;   /* workgroup shape of the sweep (block_x * block_y = 256) */
     int32_t col_planes;         /* planes walked per workgroup */
+    int32_t tiles_per_group;    /* consecutive tiles of one scene walked by one workgroup */
     int32_t candidates;         /* shapes timed */
+    int32_t reserved;
     double best_ms;             /* sweep time of the chosen shape, all scenes, one step */
 } nlos_tune;
 
