@@ -137,6 +137,26 @@ struct SourceArgs {
     float scale, inv_scale;
 };
 
+// Discrete adjoint of one two-array step (open faces at the rest state, no NSCBC).
+// state holds S'_n, the populations the step pulled from (after the source add); lam_in
+// the adjoint of S_{n+1} (what the step wrote), lam_out receives the adjoint of S'_n.
+// Both adjoint arrays are [q][ncell] in the compute type.
+struct AdjArgs {
+    const void *state;
+    const void *lam_in;
+    void *lam_out;
+    const uint8_t *mask;
+    const uint32_t *wall_bits;
+    const void *sig_x, *sig_y, *sig_z;
+    GridDesc g;
+    double omega;
+    double alpha;
+    float scale, inv_scale;
+    const long long *probe_cells;         // sorted unique probe cells
+    int n_probe_cells;
+    const double *dl_dp;                  // dL/d(rho-1) of this step per unique probe cell, or null
+};
+
 // Source entries of several scenes over a run of steps, read at the step the device counter
 // names: entries step_off[t] .. step_off[t+1] of step t = *ctr - *ctr_base.
 struct SourceMulti {

@@ -167,6 +167,9 @@ struct nlos_engine {
     std::vector<double> samples;
     bool has_source = false;
     nlos_batch *batch = nullptr;     // set when the engine is a scene of a batch
+    std::vector<DevBuf> ck;          // recorded forward: states at rec_start + i * rec_every
+    long long rec_start = -1, rec_steps = 0;
+    int rec_every = 0;
 
     dim3 block{64, 4, 1};
     int xcd_remap = 0;
@@ -344,24 +347,27 @@ int ensure_hist(nlos_engine *e, long long steps) {
 }
 
 // ---- source schedule (05:576-667, 08:543) ---------------------------------------------
-double waveform(const nlos_engine *e, double t) {
+// The waveform divided by amp (the source value is amp times this).
+double waveform_unit(const nlos_engine *e, double t) {
     const nlos_source &s = e->src;
     switch (s.waveform) {
-    case NLOS_WAVE_GAUSS_PULSE: return s.amp * std::exp(-0.5 * std::pow((t - s.t0) / s.sigma_t, 2));
-    case NLOS_WAVE_SINE: return s.amp * std::sin(2.0 * PI * s.freq * t);
+    case NLOS_WAVE_GAUSS_PULSE: return std::exp(-0.5 * std::pow((t - s.t0) / s.sigma_t, 2));
+    case NLOS_WAVE_SINE: return std::sin(2.0 * PI * s.freq * t);
     case NLOS_WAVE_SAMPLED: {
         const long long k = std::llround(t / e->dt);
         if (k < 0 || k >= (long long)e->samples.size()) return 0.0;
-        return s.amp * e->samples[(size_t)k];
+        return e->samples[(size_t)k];
     }
     case NLOS_WAVE_RICKER: {
         double a = PI * s.freq * (t - s.t0);
         a = a * a;
-        return s.amp * (1.0 - 2.0 * a) * std::exp(-a);
+        return (1.0 - 2.0 * a) * std::exp(-a);
     }
     }
     return 0.0;
 }
+
+double waveform(const nlos_engine *e, double t) { return e->src.amp * waveform_unit(e, t); }
 
 // DroneSource::update (05:221-238)
 void source_at(const nlos_source &s, double t, double &cx, double &cy, double &cz) {
@@ -376,8 +382,15 @@ void source_at(const nlos_source &s, double t, double &cx, double &cy, double &c
     cz = s.sz + r * dz;
 }
 
-// Appends the stencil cells and adds of time t.
-void schedule_step(nlos_engine *e, double t) {
+// Derivatives of the adds of one step (for the adjoint): per entry d add / d val (val the
+// waveform value) and d add / d c (c the source centre, 3 per entry).
+struct SourceDeriv {
+    std::vector<double> dval;
+    std::vector<double> dc;
+};
+
+// Appends the stencil cells and adds of time t (and their derivatives when d is set).
+void schedule_step(nlos_engine *e, double t, SourceDeriv *d = nullptr) {
     if (!e->has_source) return;
     const nlos_config &c = e->cfg;
     const nlos_source &s = e->src;
@@ -388,6 +401,7 @@ void schedule_step(nlos_engine *e, double t) {
     const int NX = e->g.nx, NY = e->g.ny, NZ = e->g.nz;
     long long cells[512];
     double ww[512];
+    double gw[512][3];   // d ww / d c
     int n = 0;
     double sumw = 0.0;
     if (s.injection == NLOS_INJECT_TRILINEAR) {
@@ -404,6 +418,12 @@ void schedule_step(nlos_engine *e, double t) {
             if (e->solid_interior(gi, gj, gk)) continue;
             sumw += wt;
             cells[n] = e->cell(gi + 1, gj + 1, gk + 1);
+            if (d) {
+                const double ax = di ? tx : 1.0 - tx, ay = dj ? ty : 1.0 - ty, az = dk ? tz : 1.0 - tz;
+                gw[n][0] = (di ? 1.0 : -1.0) / DX * ay * az;
+                gw[n][1] = (dj ? 1.0 : -1.0) / DX * ax * az;
+                gw[n][2] = (dk ? 1.0 : -1.0) / DX * ax * ay;
+            }
             ww[n++] = wt;
         }
     } else {
@@ -426,6 +446,11 @@ void schedule_step(nlos_engine *e, double t) {
                     if (e->solid_interior(gi, gj, gk)) continue;
                     sumw += w;
                     cells[n] = e->cell(gi + 1, gj + 1, gk + 1);
+                    if (d) {
+                        gw[n][0] = w * dx / sigma2;
+                        gw[n][1] = w * dy / sigma2;
+                        gw[n][2] = w * dz / sigma2;
+                    }
                     ww[n++] = w;
                 }
     }
@@ -434,6 +459,17 @@ void schedule_step(nlos_engine *e, double t) {
     for (int q = 0; q < n; ++q) {
         e->sched_cells.push_back(cells[q]);
         e->sched_adds.push_back(A * ww[q]);
+    }
+    if (d) {
+        // add_q = val dt gain ww_q / S, S = sum ww: d/dc = val dt gain (gw_q S - ww_q sum gw) / S^2
+        double sg[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < n; ++q)
+            for (int a = 0; a < 3; ++a) sg[a] += gw[q][a];
+        const double f = e->dt * s.gain / sumw;
+        for (int q = 0; q < n; ++q) {
+            d->dval.push_back(f * ww[q]);
+            for (int a = 0; a < 3; ++a) d->dc.push_back(val * f * (gw[q][a] - ww[q] * sg[a] / sumw));
+        }
     }
 }
 
@@ -790,6 +826,7 @@ NLOS_API int nlos_destroy(nlos_engine *e) {
                       &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
                       &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1], &e->wall_bits};
     for (DevBuf *b : bufs) b->release();
+    for (DevBuf &b : e->ck) b.release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -1703,4 +1740,204 @@ NLOS_API int nlos_batch_timing_get(nlos_batch *b, nlos_timing *t) {
     t->total_ms = b->timed_ms;
     t->avg_ms = b->timed_steps ? b->timed_ms / (double)b->timed_steps : 0.0;
     return NLOS_OK;
+}
+
+// =========================================================================================
+// Gradients by the discrete adjoint (k_adjoint): record, then backward over segments.
+namespace {
+
+int adjoint_supported(const nlos_engine *e) {
+    if (e->pattern != NLOS_PATTERN_TWO_ARRAY)
+        return fail(NLOS_ERR_UNSUPPORTED, "the adjoint needs the two-array stream pattern");
+    if (e->cfg.nscbc_faces != 0) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint needs nscbc_faces == 0");
+    if (e->mode == M_F16) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint supports NLOS_F64 and NLOS_F32");
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch");
+    return NLOS_OK;
+}
+
+size_t pop_bytes(const nlos_engine *e) { return (size_t)e->g.ncell * Q * e->s_bytes; }
+
+}  // namespace
+
+NLOS_API int nlos_record_forward(nlos_engine *e, int64_t n, int32_t every) {
+    if (int rc = ready(e)) return rc;
+    if (int rc = adjoint_supported(e)) return rc;
+    if (n <= 0 || every <= 0) return fail(NLOS_ERR_ARG, "steps and checkpoint interval must be positive");
+    for (DevBuf &b : e->ck) b.release();
+    e->ck.clear();
+    e->rec_start = e->step;
+    e->rec_steps = 0;
+    e->rec_every = every;
+    for (long long done = 0; done < n; done += every) {
+        DevBuf b;
+        if (b.reserve(pop_bytes(e)) != hipSuccess) {
+            e->rec_start = -1;
+            return fail(NLOS_ERR_NOMEM, "cannot allocate checkpoint %zu", e->ck.size());
+        }
+        HIP_TRY(hipMemcpyAsync(b.p, e->curp(), pop_bytes(e), hipMemcpyDeviceToDevice, e->stream));
+        e->ck.push_back(b);
+        if (int rc = nlos_step(e, std::min<long long>(every, n - done), 0)) return rc;
+        e->rec_steps += std::min<long long>(every, n - done);
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_steps, nlos_grad *grad) {
+    if (int rc = ready(e)) return rc;
+    if (int rc = adjoint_supported(e)) return rc;
+    if (!grad) return fail(NLOS_ERR_ARG, "null gradient");
+    if (e->rec_start < 0 || e->step != e->rec_start + e->rec_steps)
+        return fail(NLOS_ERR_STATE, "no recorded forward ending at the current step (nlos_record_forward)");
+    const long long s_end = e->rec_start + e->rec_steps;
+    const size_t np = e->probes.size(), nu = e->probe_ucells.size();
+    if (np > 0 && (!dl_dprobe || max_steps < s_end)) return fail(NLOS_ERR_ARG, "dl_dprobe must cover every recorded step");
+    if (int rc = check_flag(e)) return rc;
+    const int every = e->rec_every;
+    const size_t pb = pop_bytes(e);
+    const long long lam_n = e->g.ncell * (long long)Q;
+    // dL/d(rho-1) per unique probe cell and recorded step
+    std::vector<double> dl((size_t)e->rec_steps * std::max<size_t>(nu, 1), 0.0);
+    for (size_t p = 0; p < np; ++p)
+        for (long long s = 0; s < e->rec_steps; ++s)
+            dl[(size_t)s * nu + e->probes[p].unique] += dl_dprobe[p * max_steps + e->rec_start + s];
+    std::vector<DevBuf> store((size_t)every + 1);
+    DevBuf lam[2], dl_dev, src_cells, src_adds, lam_src;
+    auto release = [&]() {
+        for (DevBuf &b : store) b.release();
+        lam[0].release(); lam[1].release(); dl_dev.release(); src_cells.release(); src_adds.release();
+        lam_src.release();
+    };
+    auto bail = [&](int rc) { release(); return rc; };
+    for (DevBuf &b : store)
+        if (b.reserve(pb) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "cannot allocate %d recompute states", every + 1));
+    if (lam[0].reserve(lam_n * e->c_bytes) != hipSuccess || lam[1].reserve(lam_n * e->c_bytes) != hipSuccess)
+        return bail(fail(NLOS_ERR_NOMEM, "cannot allocate adjoint arrays"));
+    if (nu > 0) {
+        if (dl_dev.reserve(dl.size() * 8) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "probe adjoint"));
+        if (hipMemcpy(dl_dev.p, dl.data(), dl.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(NLOS_ERR_HIP, "probe adjoint upload"));
+    }
+    auto hip = [&](hipError_t r, const char *what) {
+        return r == hipSuccess ? NLOS_OK : fail(NLOS_ERR_HIP, "%s: %s", what, hipGetErrorString(r));
+    };
+    int rc = hip(by_mode(e->mode, [&](auto L) { return decltype(L)::fill_compute(lam[0].p, lam_n, e->stream); }),
+                 "adjoint init");
+    if (rc) return bail(rc);
+    double g_pos[3] = {0.0, 0.0, 0.0}, g_amp = 0.0;
+    std::vector<double> g_samp(e->samples.size(), 0.0);
+    int cur = 0;
+    const int nseg = (int)e->ck.size();
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const long long s0 = e->rec_start + (long long)seg * every;
+        const long long s1 = std::min<long long>(s0 + every, s_end);
+        const int len = (int)(s1 - s0);
+        // source entries of the segment with their derivatives
+        std::vector<long long> off(1, 0);
+        std::vector<double> unit;
+        SourceDeriv d;
+        e->sched_cells.clear();
+        e->sched_adds.clear();
+        for (long long n = s0; n < s1; ++n) {
+            schedule_step(e, (double)n * e->dt, &d);
+            off.push_back((long long)e->sched_cells.size());
+            unit.push_back(waveform_unit(e, (double)n * e->dt));
+        }
+        const size_t ne = e->sched_cells.size();
+        if (ne > 0) {
+            if (src_cells.reserve(ne * 8) != hipSuccess || src_adds.reserve(ne * 8) != hipSuccess ||
+                lam_src.reserve(ne * 8) != hipSuccess)
+                return bail(fail(NLOS_ERR_NOMEM, "source entries"));
+            if ((rc = hip(hipMemcpyAsync(src_cells.p, e->sched_cells.data(), ne * 8, hipMemcpyHostToDevice, e->stream), "upload")) ||
+                (rc = hip(hipMemcpyAsync(src_adds.p, e->sched_adds.data(), ne * 8, hipMemcpyHostToDevice, e->stream), "upload")))
+                return bail(rc);
+        }
+        // recompute the segment: store[k] = S_{s0+k} (after the source add of that step)
+        if ((rc = hip(hipMemcpyAsync(store[0].p, e->ck[seg].p, pb, hipMemcpyDeviceToDevice, e->stream), "checkpoint copy")))
+            return bail(rc);
+        for (int k = 0; k < len; ++k) {
+            SourceArgs sa = source_args(e, 0);
+            sa.pop = store[k].p;
+            sa.fb[0] = sa.fb[1] = nullptr;
+            sa.cells = src_cells.as<long long>() + off[k];
+            sa.adds = src_adds.as<double>() + off[k];
+            sa.n = (int)(off[k + 1] - off[k]);
+            GhostArgs ga = ghost_args(e, 0);
+            ga.pop = store[k].p;
+            ga.fb[0] = ga.fb[1] = nullptr;
+            StepArgs st = step_args(e, 0, e->block, e->col_planes);
+            st.pop = store[k].p;
+            st.pop_out = store[k + 1].p;
+            st.n_probe_cells = 0;
+            st.fb[0] = st.fb[1] = nullptr;
+            rc = hip(by_mode(e->mode, [&](auto L) {
+                using LL = decltype(L);
+                hipError_t r = LL::source(sa, LAY_AB, e->stream);
+                if (r == hipSuccess) r = LL::ghost(ga, LAY_AB, e->stream);
+                if (r == hipSuccess) r = LL::step_ab(st, false, e->block, e->stream);
+                return r;
+            }), "recompute");
+            if (rc) return bail(rc);
+        }
+        // adjoint steps back to the segment start
+        for (int k = len - 1; k >= 0; --k) {
+            const long long n = s0 + k;
+            AdjArgs a{};
+            a.state = store[k].p;
+            a.lam_in = lam[cur].p;
+            a.lam_out = lam[cur ^ 1].p;
+            a.mask = e->mask.as<uint8_t>();
+            a.wall_bits = e->wall_bits.as<uint32_t>();
+            a.sig_x = e->sig[0].p;
+            a.sig_y = e->sig[1].p;
+            a.sig_z = e->sig[2].p;
+            a.g = e->g;
+            a.omega = e->omega;
+            a.alpha = e->cfg.wall_reflect;
+            a.scale = e->scale;
+            a.inv_scale = e->inv_scale;
+            a.probe_cells = e->probe_cells.as<long long>();
+            a.n_probe_cells = (int)nu;
+            a.dl_dp = nu ? dl_dev.as<double>() + (size_t)(n - e->rec_start) * nu : nullptr;
+            const int cnt = (int)(off[k + 1] - off[k]);
+            rc = hip(by_mode(e->mode, [&](auto L) {
+                using LL = decltype(L);
+                hipError_t r = LL::adjoint(a, e->stream);
+                if (r == hipSuccess && cnt > 0)
+                    r = LL::source_dot(lam[cur ^ 1].p, e->g.ncell, src_cells.as<long long>() + off[k], cnt,
+                                       lam_src.as<double>() + off[k], e->stream);
+                return r;
+            }), "adjoint step");
+            if (rc) return bail(rc);
+            cur ^= 1;
+        }
+        // gradient of the segment's source terms (FP64, host)
+        if (ne > 0) {
+            std::vector<double> L(ne);
+            if ((rc = hip(hipMemcpyAsync(L.data(), lam_src.p, ne * 8, hipMemcpyDeviceToHost, e->stream), "download")) ||
+                (rc = hip(hipStreamSynchronize(e->stream), "sync")))
+                return bail(rc);
+            for (int k = 0; k < len; ++k) {
+                const long long n = s0 + k;
+                double dv = 0.0;
+                for (long long q = off[k]; q < off[k + 1]; ++q) {
+                    dv += L[q] * d.dval[q];
+                    for (int ax = 0; ax < 3; ++ax) g_pos[ax] += L[q] * d.dc[3 * q + ax];
+                }
+                g_amp += dv * unit[k];
+                if (e->src.waveform == NLOS_WAVE_SAMPLED) {
+                    const long long si = std::llround((double)n * e->dt / e->dt);
+                    if (si >= 0 && si < (long long)g_samp.size()) g_samp[(size_t)si] += dv * e->src.amp;
+                }
+            }
+        }
+    }
+    if ((rc = hip(hipStreamSynchronize(e->stream), "sync"))) return bail(rc);
+    release();
+    for (int a = 0; a < 3; ++a) grad->d_position[a] = g_pos[a];
+    grad->d_amp = g_amp;
+    if (grad->d_samples)
+        for (int64_t k = 0; k < grad->n_samples; ++k)
+            grad->d_samples[k] = k < (int64_t)g_samp.size() ? g_samp[(size_t)k] : 0.0;
+    return check_flag(e);
 }

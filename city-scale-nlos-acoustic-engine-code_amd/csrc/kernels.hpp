@@ -694,6 +694,124 @@ __global__ __launch_bounds__(256) void k_prime_walls(void *pop, const uint8_t *m
     }
 }
 
+// ---- discrete adjoint of the two-array step -----------------------------------------------
+// Forward, per fluid cell x: f_q = S'_q(x - e_q), or the reflected S'_{INV q}(x) when x - e_q
+// is solid; f* = (1 - sigma) [f - omega (f - feq(rho, u))] + sigma w; S_{n+1,q}(x) = f*_q;
+// probe rho - 1 = sum f.  The transpose, per fluid cell x:
+//   mu_p = (1 - sigma) [(1 - omega) lam_p + omega (A (1 + 1.5 u^2) - 4.5 D
+//          + e_p . (3 B + 9 C - 3 A u))] (+ dL/dp at probe cells)
+// with A = sum w_q lam_q, B = sum w_q lam_q e_q, C = sum w_q lam_q (e_q . u) e_q,
+// D = sum w_q lam_q (e_q . u)^2 (the derivative of feq_q = w_q (rho + 3 e.j + 4.5 (e.j)^2/rho
+// - 1.5 j^2/rho) with respect to f_p), and mu_q goes back to the slot it was pulled from:
+// slot q of x - e_q, or (times alpha) slot INV[q] of x.  Every slot of lam_out has exactly
+// one writer: the pull is a one-to-one map of slots; slots that only halo cells read are
+// set to zero by their own cell (the halo holds the rest state, a constant).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
+    using C = typename Tr<MODE>::C;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const GridDesc &g = a.g;
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    if (n >= total) return;
+    const int i = (int)(n % g.nx) + 1;
+    const int j = (int)((n / g.nx) % g.ny) + 1;
+    const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
+    const long long c = cell_index(g, i, j, k);
+    const long long N = g.ncell;
+    C *lo = static_cast<C *>(a.lam_out);
+    const uint8_t m = a.mask[c];
+    if (m & MK_SOLID) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) lo[(long long)q * N + c] = C(0);
+        return;
+    }
+    const uint32_t sb = (m & MK_NEAR) ? a.wall_bits[c] : 0u;   // bit q: c - e_q is solid
+    // pulled populations (full values) and moments
+    C f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        C v;
+        if ((sb >> q) & 1u) {
+            v = ld<MODE>(a.state, (long long)INV[q] * N + c, a.inv_scale);
+            if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
+        } else {
+            v = ld<MODE>(a.state, (long long)q * N + c - nb_off(g, q), a.inv_scale);
+        }
+        if constexpr (MODE != M_F64) v += wq<C>(q);   // deviation storage
+        f[q] = v;
+    }
+    C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        rho += f[q];
+        jx += f[q] * (C)EX[q];
+        jy += f[q] * (C)EY[q];
+        jz += f[q] * (C)EZ[q];
+    }
+    const C ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    const C u2 = ux * ux + uy * uy + uz * uz;
+    const C sig = fmax(static_cast<const C *>(a.sig_x)[i],
+                       fmax(static_cast<const C *>(a.sig_y)[j], static_cast<const C *>(a.sig_z)[k]));
+    const C *li = static_cast<const C *>(a.lam_in);
+    C lam[Q];
+    C A = C(0), Bx = C(0), By = C(0), Bz = C(0), Cx = C(0), Cy = C(0), Cz = C(0), D = C(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        lam[q] = li[(long long)q * N + c];
+        const C wl = wq<C>(q) * lam[q];
+        const C eu = (C)EX[q] * ux + (C)EY[q] * uy + (C)EZ[q] * uz;
+        A += wl;
+        Bx += wl * (C)EX[q]; By += wl * (C)EY[q]; Bz += wl * (C)EZ[q];
+        Cx += wl * eu * (C)EX[q]; Cy += wl * eu * (C)EY[q]; Cz += wl * eu * (C)EZ[q];
+        D += wl * eu * eu;
+    }
+    const C om = (C)a.omega, keep = C(1) - sig;
+    const C base = A * (C(1) + C(1.5) * u2) - C(4.5) * D;
+    const C vx = C(3) * Bx + C(9) * Cx - C(3) * A * ux;
+    const C vy = C(3) * By + C(9) * Cy - C(3) * A * uy;
+    const C vz = C(3) * Bz + C(9) * Cz - C(3) * A * uz;
+    C probe = C(0);
+    if ((m & MK_PROBE) && a.dl_dp) {
+        const int u = probe_slot(a.probe_cells, a.n_probe_cells, c);
+        if (u >= 0) probe = (C)a.dl_dp[u];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const C mu = keep * ((C(1) - om) * lam[q] + om * (base + (C)EX[q] * vx + (C)EY[q] * vy + (C)EZ[q] * vz)) + probe;
+        if ((sb >> q) & 1u) {
+            lo[(long long)INV[q] * N + c] = (C)a.alpha * mu;
+        } else {
+            const int si = i - EX[q], sj = j - EY[q], sk = k - EZ[q];
+            if (si >= 1 && si <= g.nx && sj >= 1 && sj <= g.ny && sk >= 1 && sk <= g.nz)
+                lo[(long long)q * N + c - nb_off(g, q)] = mu;
+        }
+        // slot q of c is read only by the halo cell c + e_q (a constant): zero it
+        const int ti = i + EX[q], tj = j + EY[q], tk = k + EZ[q];
+        if (q != 0 && (ti < 1 || ti > g.nx || tj < 1 || tj > g.ny || tk < 1 || tk > g.nz) &&
+            !(a.mask[c + nb_off(g, q)] & MK_SOLID))
+            lo[(long long)q * N + c] = C(0);
+    }
+}
+
+// out[e] = sum_q w_q lam[q][cells[e]] for the n source entries of one step.
+template <class T>
+__global__ __launch_bounds__(256) void k_zero(T *p, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = T(0);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_source_dot(const void *lam, long long ncell, const long long *cells, int n,
+                                                    double *out) {
+    using C = typename Tr<MODE>::C;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const C *l = static_cast<const C *>(lam);
+    double acc = 0.0;
+    for (int q = 0; q < Q; ++q) acc += W64[q] * (double)l[(long long)q * ncell + cells[e]];
+    out[e] = acc;
+}
+
 // ---- reading f* of an interior cell in the current layout ---------------------------------
 
 template <int MODE, int LAY>
@@ -1185,6 +1303,29 @@ template <int MODE>
 hipError_t Launch<MODE>::source_multi(const SourceMulti &m, long long max_entries, hipStream_t s) {
     if (max_entries <= 0) return hipSuccess;
     k_source_multi<MODE, LAY_AB><<<blocks_for(max_entries * Q, 256), 256, 0, s>>>(m);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::adjoint(const AdjArgs &a, hipStream_t s) {
+    const long long total = (long long)a.g.nx * a.g.ny * a.g.nz;
+    k_adjoint<MODE><<<blocks_for(total, 256), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::source_dot(const void *lam, long long ncell, const long long *cells, int n, double *out,
+                                    hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_source_dot<MODE><<<blocks_for(n, 256), 256, 0, s>>>(lam, ncell, cells, n, out);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::fill_compute(void *p, long long n, hipStream_t s) {
+    using C = typename Tr<MODE>::C;
+    if (n <= 0) return hipSuccess;
+    k_zero<C><<<blocks_for(n, 256), 256, 0, s>>>(static_cast<C *>(p), n);
     return hipGetLastError();
 }
 

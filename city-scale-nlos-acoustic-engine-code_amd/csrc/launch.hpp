@@ -44,6 +44,13 @@ struct Launch {
     static hipError_t solid_multi(const SolidArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
     // Source entries of the step named by the device counter; at most max_entries per step.
     static hipError_t source_multi(const SourceMulti &m, long long max_entries, hipStream_t s);
+    // Discrete adjoint of one two-array step (see AdjArgs).
+    static hipError_t adjoint(const AdjArgs &a, hipStream_t s);
+    // out[e] = sum_q w_q lam[q][cells[e]], e < n (FP64).
+    static hipError_t source_dot(const void *lam, long long ncell, const long long *cells, int n, double *out,
+                                 hipStream_t s);
+    // n values of the compute type set to zero.
+    static hipError_t fill_compute(void *p, long long n, hipStream_t s);
     // Gather f* of every interior cell into AoS FP64 full populations [cell][q] over the
     // interior, x fastest (for checkpoints and tests).
     static hipError_t gather_f(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,

@@ -69,6 +69,11 @@ class Timing(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("avg_ms", C.c_double)]
 
 
+class Grad(C.Structure):
+    _fields_ = [("d_position", C.c_double * 3), ("d_amp", C.c_double),
+                ("d_samples", C.POINTER(C.c_double)), ("n_samples", C.c_int64)]
+
+
 class Tune(C.Structure):
     _fields_ = [("block_x", C.c_int32), ("block_y", C.c_int32), ("col_planes", C.c_int32),
                 ("tiles_per_group", C.c_int32), ("candidates", C.c_int32), ("reserved", C.c_int32),
@@ -108,6 +113,8 @@ SIGNATURES = {
     "nlos_timing_enable": (C.c_int, [_E, C.c_int32]),
     "nlos_timing_reset": (C.c_int, [_E]),
     "nlos_timing_get": (C.c_int, [_E, C.POINTER(Timing)]),
+    "nlos_record_forward": (C.c_int, [_E, C.c_int64, C.c_int32]),
+    "nlos_backward": (C.c_int, [_E, _P, C.c_int64, C.POINTER(Grad)]),
     "nlos_batch_create": (C.c_int, [C.POINTER(Config), C.c_int32, C.POINTER(C.c_void_p)]),
     "nlos_batch_destroy": (C.c_int, [_P]),
     "nlos_batch_size": (C.c_int, [_P, C.POINTER(C.c_int32)]),
@@ -358,6 +365,26 @@ class Engine:
 
     def timing_reset(self) -> None:
         _check(self.lib.nlos_timing_reset(self.h))
+
+    # -- gradients -------------------------------------------------------------------
+    def record_forward(self, n: int, every: int) -> None:
+        """n steps like step(n), keeping a checkpoint every `every` steps for backward()."""
+        _check(self.lib.nlos_record_forward(self.h, int(n), int(every)))
+
+    def backward(self, dl_dprobe: np.ndarray, n_samples: int = 0) -> dict:
+        """Gradient of a loss over the recorded steps from dL/d(rho-1) per probe and step
+        (array [probe][step] over all steps so far): d_position (whole-path shift, per
+        metre), d_amp, and d_samples for a sampled waveform."""
+        dl = np.ascontiguousarray(dl_dprobe, np.float64)
+        g = Grad()
+        samples = np.zeros(max(n_samples, 1), np.float64)
+        if n_samples:
+            g.d_samples = samples.ctypes.data_as(C.POINTER(C.c_double))
+            g.n_samples = n_samples
+        steps = dl.shape[1] if dl.ndim == 2 else 1
+        _check(self.lib.nlos_backward(self.h, _ptr(dl), steps, C.byref(g)))
+        return {"d_position": np.array(list(g.d_position)), "d_amp": g.d_amp,
+                "d_samples": samples[:n_samples].copy()}
 
     def debug_read_f(self) -> np.ndarray:
         nx, ny, nz = self.dims

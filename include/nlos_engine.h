@@ -210,6 +210,35 @@ int nlos_timing_enable(nlos_engine *e, int32_t on);
 int nlos_timing_reset(nlos_engine *e);
 int nlos_timing_get(nlos_engine *e, nlos_timing *t);
 
+/* ---- gradients (discrete adjoint) ------------------------------------------------------
+ * No counterpart in the reference, which has no differentiation.  The adjoint transposes
+ * the two-array step exactly (pull with half-way bounce-back and wall reflection, BGK
+ * collision, sponge blend, probe record, source injection) for open faces held at the rest
+ * state: it needs stream_pattern NLOS_PATTERN_TWO_ARRAY, nscbc_faces == 0 and precision
+ * NLOS_F64 or NLOS_F32.
+ *
+ * nlos_record_forward runs n steps like nlos_step and keeps the state at its first step and
+ * every `every` steps after (device memory, one population array each).
+ * nlos_backward then returns the gradient of a loss L over the recorded steps, given
+ * dL/d(rho-1) for every probe and step ([probe][step], stride max_steps, global step
+ * indices as in nlos_read_probes).  It recomputes each segment of `every` steps from its
+ * checkpoint (every + 1 population arrays of device memory), runs the adjoint step by step
+ * back to the segment start, and accumulates in FP64:
+ *   d_position: dL/d(shift of the whole source path, start and end together), per metre;
+ *   d_amp:      dL/d amp;
+ *   d_samples:  for NLOS_WAVE_SAMPLED, dL/d samples[k] (caller array of n_samples, or NULL).
+ * The stencil cell set of the injection is piecewise constant in the position; the
+ * gradient is the derivative within the current set.  The engine state is not changed. */
+typedef struct nlos_grad {
+    double d_position[3];
+    double d_amp;
+    double *d_samples;
+    int64_t n_samples;
+} nlos_grad;
+
+int nlos_record_forward(nlos_engine *e, int64_t n, int32_t every);
+int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_steps, nlos_grad *grad);
+
 /* ---- batched scenes ------------------------------------------------------------------
  * No counterpart in the reference, which steps one domain per process (05:1168-1170).  A
  * batch owns n independent scenes, each an nlos_engine of its own configuration (same

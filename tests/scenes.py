@@ -70,3 +70,31 @@ def rel_l2(a, b):
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     n = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def canyon_scene(precision="f64", nx=28, ny=24, nz=14, waveform="ricker", n_samples=0):
+    """Street canyon for the adjoint: two building rows along x with a 6-cell street, ground,
+    sponge on five faces, open faces at the rest state (no NSCBC), partially reflecting walls,
+    a static Gaussian-stencil source (sigma 2 cells, radius 3) in the street, four probes."""
+    cfg = nlos.default_config(dx=1.0, x_min=0.0, x_max=float(nx), y_min=0.0, y_max=float(ny),
+                              z_min=0.0, z_max=float(nz), sponge_width=4.0, sponge_strength=0.2,
+                              tau=0.6, sky_z=1e9, wall_reflect=0.9, nscbc_faces=0,
+                              stream_pattern="two_array", precision=precision)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    c = ny // 2
+    solid[:7, c - 3 - 5:c - 3, 4:nx - 4] = 1
+    solid[:9, c + 3:c + 3 + 5, 6:nx - 6] = 1
+    kw = dict(sx=11.37, sy=c + 0.21, sz=3.63, ex=11.37, ey=c + 0.21, ez=3.63, speed=0.0, amp=1.0,
+              stencil_sigma_cells=2.0, stencil_radius=3, injection=nlos.INJECT_GAUSS5)
+    if waveform == "sampled":
+        t = np.arange(n_samples) * nlos.dt_of(cfg)
+        f0, t0 = 90.0, 0.012
+        a = (np.pi * f0 * (t - t0)) ** 2
+        samples = (1.0 - 2.0 * a) * np.exp(-a)
+        src = nlos.default_source(waveform=nlos.WAVE_SAMPLED, **kw)
+        src_samples = samples
+    else:
+        src = nlos.default_source(waveform=nlos.WAVE_RICKER, freq=90.0, t0=0.012, **kw)
+        src_samples = None
+    probes = [(3.5, c + 0.5, 1.5), (nx - 3.5, c - 1.5, 2.5), (15.5, c + 1.5, 10.5), (20.2, c - 0.4, 4.4)]
+    return cfg, solid, src, src_samples, probes
