@@ -308,6 +308,120 @@ def bench_batch(args) -> None:
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args) -> None:
+    """Gradient of the probe misfit L = sum (rho-1 - obs)^2 over 3000 steps with respect to
+    the source position, amplitude and 3000 waveform samples, in the urban canyon: FP32
+    forward with a checkpoint every 100 steps, discrete adjoint, FP64 gradient accumulation.
+    Reports forward, recorded-forward and backward times and checks the gradient against
+    central differences (FP64 forward, fourth order; and FP32 forward, second order at the
+    stated steps h = 1e-3 cells and 1e-6 amplitude)."""
+    steps = args.steps if args.steps != 200 else 3000
+    every = 100
+    wl = workloads.CanyonWorkload(n_samples=steps)
+    stl = workloads.stl_bytes(wl.boxes())
+    samples = wl.samples()
+    probes = wl.probes()
+
+    def engine(precision, src, samp):
+        e = nlos.Engine(wl.config(precision))
+        e.voxelize_stl(stl)
+        e.set_source(src, samp)
+        e.set_probes(probes)
+        return e
+
+    # observation: a seeded true source (offset position, other amplitude) plus 1 % RMS noise
+    rng = np.random.Generator(np.random.PCG64(5))
+    true_src = wl.source(x=0.41 * wl.nx + 4.13, y=wl.ny / 2.0 - 1.37, z=5.21, amp=1.3)
+    e = engine("f32", true_src, samples)
+    e.step(steps)
+    clean = e.read_probes()
+    e.close()
+    obs = clean + 0.01 * np.sqrt(np.mean(clean ** 2)) * rng.standard_normal(clean.shape)
+    src = wl.source()
+
+    def loss(precision, s=None, samp=None):
+        e = engine(precision, s if s is not None else src, samp if samp is not None else samples)
+        e.step(steps)
+        r = e.read_probes() - obs
+        e.close()
+        return float(np.sum(r * r))
+
+    e = engine("f32", src, samples)
+    e.step(2)
+    e.synchronize()
+    e.close()
+    e = engine("f32", src, samples)
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.synchronize()
+    t_fwd = time.perf_counter() - t0
+    e.close()
+    e = engine("f32", src, samples)
+    t0 = time.perf_counter()
+    e.record_forward(steps, every)
+    e.synchronize()
+    t_rec = time.perf_counter() - t0
+    r = e.read_probes() - obs
+    L = float(np.sum(r * r))
+    t0 = time.perf_counter()
+    g = e.backward(2.0 * r, n_samples=len(samples))
+    t_bwd = time.perf_counter() - t0
+    e.close()
+    log(f"forward {t_fwd:.2f} s, recorded forward {t_rec:.2f} s, backward {t_bwd:.2f} s")
+
+    def moved(d=(0.0, 0.0, 0.0), amp=None):
+        s = wl.source(x=src.sx + d[0], y=src.sy + d[1], z=src.sz + d[2], amp=src.amp if amp is None else amp)
+        return s
+
+    def fd4(fun, h):
+        return (8.0 * (fun(h) - fun(-h)) - (fun(2 * h) - fun(-2 * h))) / (12.0 * h)
+
+    fd64 = {"d_position": [], "d_amp": None}
+    for ax in range(3):
+        def f(x, ax=ax):
+            d = [0.0, 0.0, 0.0]
+            d[ax] = x
+            return loss("f64", moved(d))
+        fd64["d_position"].append(fd4(f, 1e-3))
+    fd64["d_amp"] = fd4(lambda x: loss("f64", moved(amp=src.amp + x)), 1e-2 * src.amp)
+    k = int(np.argmax(np.abs(g["d_samples"])))
+
+    def fs(x):
+        sx = samples.copy()
+        sx[k] += x
+        return loss("f64", samp=sx)
+    fd64_sample = fd4(fs, 1e-2)
+    # the stated FP32 central differences
+    h = 1e-3
+    fd32_x = (loss("f32", moved((h, 0.0, 0.0))) - loss("f32", moved((-h, 0.0, 0.0)))) / (2 * h)
+    ha = 1e-6
+    fd32_amp = (loss("f32", moved(amp=src.amp + ha)) - loss("f32", moved(amp=src.amp - ha))) / (2 * ha)
+    gp = np.asarray(g["d_position"])
+    fp = np.asarray(fd64["d_position"])
+    out = {"metric": "discrete-adjoint gradient cost and accuracy, urban canyon source localisation",
+           "value": round(t_bwd / t_fwd, 3), "unit": "backward time / forward time", "n_gpus": 1,
+           "steps": steps, "warmup": 2, "ms_per_step": round(1e3 * t_fwd / steps, 4),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": f"canyon {wl.nx}x{wl.ny}x{wl.nz}, street {wl.street} cells, "
+                                  f"{len(wl.boxes())} seeded buildings, 16 probes, sampled Ricker "
+                                  f"{steps} samples, checkpoint every {every} steps"},
+           "forward_s": round(t_fwd, 3), "recorded_forward_s": round(t_rec, 3), "backward_s": round(t_bwd, 3),
+           "gradient_total_over_forward": round((t_rec + t_bwd) / t_fwd, 3),
+           "loss": L,
+           "grad": {"d_position": gp.tolist(), "d_amp": g["d_amp"], "d_sample_max_index": k,
+                    "d_sample_max": float(g["d_samples"][k])},
+           "fd64_4th_order": {"d_position": fp.tolist(), "d_amp": fd64["d_amp"], "d_sample": fd64_sample,
+                              "h_position_cells": 1e-3, "h_amp": 1e-2 * src.amp, "h_sample": 1e-2},
+           "rel_err_vs_fd64": {"d_position_vector": float(np.linalg.norm(gp - fp) / np.linalg.norm(fp)),
+                               "d_amp": abs(g["d_amp"] - fd64["d_amp"]) / abs(fd64["d_amp"]),
+                               "d_sample": abs(g["d_samples"][k] - fd64_sample) / abs(fd64_sample)},
+           "fd32_central": {"d_position_x": fd32_x, "d_amp": fd32_amp, "h_position_cells": h, "h_amp": ha,
+                            "rel_err_x": abs(gp[0] - fd32_x) / abs(fd32_x) if fd32_x else None,
+                            "rel_err_amp": abs(g["d_amp"] - fd32_amp) / abs(fd32_amp) if fd32_amp else None}}
+    print(json.dumps(out), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,11 +437,12 @@ def main() -> None:
     ap.add_argument("--pattern", default="auto", choices=["auto", "inplace", "two_array"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 2 GiB device copy kernel (PMC byte calibration)")
-    ap.add_argument("--workload", default="city", choices=["city", "scatter", "city2048", "batch"],
+    ap.add_argument("--workload", default="city", choices=["city", "scatter", "city2048", "batch", "adjoint"],
                     help="city: the headline moving-source city step (default); scatter: the "
                          "128^3 rigid-cube case, FP64 against the oracle; city2048: the dx=0.5 m "
                          "precision sweep on 2048x2048x512 with its 512x512x256 error sub-box; "
-                         "batch: 256 small FP32 scenes advanced together")
+                         "batch: 256 small FP32 scenes advanced together; adjoint: source-"
+                         "localisation gradient in a 256x256x128 canyon against finite differences")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -341,6 +456,8 @@ def main() -> None:
         return bench_city2048(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     wl = workloads.CityWorkload(nx=args.nx, ny=args.ny, nz=args.nz)
     t = time.perf_counter()

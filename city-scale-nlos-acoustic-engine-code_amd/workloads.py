@@ -226,3 +226,67 @@ class BatchWorkload:
                            float(rng.uniform(self.sponge_cells, ny - self.sponge_cells)), 1.5))
             out.append(BatchScene(cfg, src, np.array(pr), (nx, ny, nz)))
         return out
+
+
+@dataclass
+class CanyonWorkload:
+    """Adjoint source localisation in an urban canyon: nx x ny x nz cells at dx = 1 m, two
+    parallel rows of seeded buildings along x (footprint lengths uniform 10-40 m, depth
+    uniform 15-30 m, heights lognormal(ln 20, 0.5) capped at nz - 20) on either side of a
+    street of `street` cells, rigid ground, 16-cell sponge on five faces, open faces at the
+    rest state (no NSCBC, as the adjoint needs).  The source is a static point source in the
+    street, spread by a Gaussian stencil of sigma 2 cells (radius 3), driven by a sampled
+    Ricker waveform (peak 100 Hz) of n_samples samples; 16 probes at 1.5 m in the street."""
+    nx: int = 256
+    ny: int = 256
+    nz: int = 128
+    street: int = 20
+    seed: int = 0
+    n_samples: int = 3000
+    n_probes: int = 16
+    tau: float = 0.5 + 1e-2
+
+    def config(self, precision="f32") -> nlos.Config:
+        return nlos.default_config(x_min=0.0, x_max=float(self.nx), y_min=0.0, y_max=float(self.ny),
+                                   z_min=0.0, z_max=float(self.nz), dx=1.0, c_phys=343.0, tau=self.tau,
+                                   sponge_width=16.0, sponge_strength=0.1, sky_z=1e9, nscbc_faces=0,
+                                   stream_pattern="two_array", precision=precision,
+                                   world_offset=(0.0, 0.0, 0.0))
+
+    def boxes(self) -> List[Box]:
+        rng = np.random.Generator(np.random.PCG64(self.seed))
+        yc = self.ny / 2.0
+        out = []
+        for side in (-1, 1):
+            x = 16.0
+            while x < self.nx - 16.0:
+                length = float(rng.uniform(10.0, 40.0))
+                depth = float(rng.uniform(15.0, 30.0))
+                h = min(self.nz - 20.0, float(np.exp(rng.normal(math.log(20.0), 0.5))))
+                x1 = min(x + length, self.nx - 16.0)
+                if side < 0:
+                    y0, y1 = yc - self.street / 2.0 - depth, yc - self.street / 2.0
+                else:
+                    y0, y1 = yc + self.street / 2.0, yc + self.street / 2.0 + depth
+                out.append((x, x1, y0, y1, 0.0, h))
+                x = x1 + float(rng.uniform(2.0, 8.0))
+        return out
+
+    def samples(self, f0=100.0, t0=0.015) -> np.ndarray:
+        t = np.arange(self.n_samples) * nlos.dt_of(self.config())
+        a = (np.pi * f0 * (t - t0)) ** 2
+        return (1.0 - 2.0 * a) * np.exp(-a)
+
+    def source(self, x=None, y=None, z=3.37, amp=1.0) -> nlos.Source:
+        x = 0.41 * self.nx + 0.27 if x is None else x
+        y = self.ny / 2.0 + 2.21 if y is None else y
+        return nlos.default_source(sx=x, sy=y, sz=z, ex=x, ey=y, ez=z, speed=0.0, amp=amp,
+                                   stencil_sigma_cells=2.0, stencil_radius=3,
+                                   injection=nlos.INJECT_GAUSS5, waveform=nlos.WAVE_SAMPLED)
+
+    def probes(self) -> np.ndarray:
+        rng = np.random.Generator(np.random.PCG64(self.seed + 7))
+        yc = self.ny / 2.0
+        xs = rng.uniform(24.0, self.nx - 24.0, self.n_probes)
+        ys = rng.uniform(yc - self.street / 2.0 + 1.0, yc + self.street / 2.0 - 1.0, self.n_probes)
+        return np.stack([xs, ys, np.full(self.n_probes, 1.5)], 1)
