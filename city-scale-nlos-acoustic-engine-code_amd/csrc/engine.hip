@@ -174,7 +174,6 @@ struct nlos_engine {
     dim3 block{64, 4, 1};
     int xcd_remap = 0;
     int use_dpp = 1;
-    int col_planes = 16;             // two-array sweep: planes per workgroup column
 
     bool timing = false;
     std::vector<hipEvent_t> ev;     // start/stop pairs
@@ -542,7 +541,7 @@ SolidArgs solid_args(const nlos_engine *e, int phase) {
     return s;
 }
 
-StepArgs step_args(const nlos_engine *e, int phase, dim3 block, int col_planes) {
+StepArgs step_args(const nlos_engine *e, int phase, dim3 block) {
     StepArgs a{};
     const bool two = e->pattern == NLOS_PATTERN_TWO_ARRAY;
     a.pop = two ? pop_at(e, phase) : e->pop.p;
@@ -561,8 +560,7 @@ StepArgs step_args(const nlos_engine *e, int phase, dim3 block, int col_planes) 
     a.flag = e->flag.as<int>();
     a.tiles_x = (e->g.nx + (int)block.x - 1) / (int)block.x;
     a.tiles_y = (e->g.ny + (int)block.y - 1) / (int)block.y;
-    a.kc = two ? std::max(1, std::min(col_planes, e->g.nz)) : 1;
-    a.nblocks = a.tiles_x * a.tiles_y * ((e->g.nz + a.kc - 1) / a.kc);
+    a.nblocks = a.tiles_x * a.tiles_y * e->g.nz;
     a.xcd_remap = e->xcd_remap;
     // lane shifts need full waves: rows of exactly 64-cell tiles, no partial block rows
     a.dpp = e->use_dpp && block.x == 64 && e->g.nx % 64 == 0 && e->g.ny % (int)block.y == 0;
@@ -613,7 +611,7 @@ int launch_sweep(nlos_engine *e, bool capture) {
         }
         HIP_TRY(launch_fill_f32(e->field.as<float>(), e->n_interior(), 0.0f, e->stream));
     }
-    StepArgs a = step_args(e, 0, e->block, e->col_planes);
+    StepArgs a = step_args(e, 0, e->block);
     a.probe_row = a.n_probe_cells ? e->hist.as<double>() + (size_t)e->step * a.n_probe_cells : nullptr;
     a.field = capture ? e->field.as<float>() : nullptr;
     const bool two = e->pattern == NLOS_PATTERN_TWO_ARRAY;
@@ -753,7 +751,6 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     const char *block_env = std::getenv("NLOS_BLOCK");
     if (const char *x = std::getenv("NLOS_XCD_REMAP")) e->xcd_remap = std::atoi(x) != 0;
     if (const char *x = std::getenv("NLOS_DPP")) e->use_dpp = std::atoi(x) != 0;
-    if (const char *x = std::getenv("NLOS_COLUMN")) e->col_planes = std::max(1, std::atoi(x));
 
     auto bail = [&](int rc) { nlos_destroy(e); return rc; };
     hipError_t he = hipGetDevice(&e->device);
@@ -773,12 +770,8 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         return bail(fail(NLOS_ERR_ARG, "bad stream_pattern %d", cfg->stream_pattern));
     e->lay = e->pattern == NLOS_PATTERN_TWO_ARRAY ? LAY_AB : LAY_P;
     // Measured on the 1024x1024x256 city: the pull sweep runs fastest on 256-cell row
-    // segments walking 4 planes, the in-place sweep on 64x4 tiles (its lane shifts need
-    // 64-cell rows).
-    if (e->pattern == NLOS_PATTERN_TWO_ARRAY) {
-        e->block = dim3(256, 1, 1);
-        if (!std::getenv("NLOS_COLUMN")) e->col_planes = 4;
-    }
+    // segments, the in-place sweep on 64x4 tiles (its lane shifts need 64-cell rows).
+    if (e->pattern == NLOS_PATTERN_TWO_ARRAY) e->block = dim3(256, 1, 1);
     if (block_env) parse_block(block_env, e->block);
     if (e->pop.reserve(popb) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate %.2f GB of populations", popb / 1e9));
@@ -1320,7 +1313,6 @@ struct nlos_batch {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     dim3 block{256, 1, 1};
-    int col_planes = 4;
     int per_group = 4;                    // tiles walked by one sweep workgroup
     DevBuf step_args[2], ghost_args[2], solid_args[2], source_args[2];
     DevBuf tile_table, halo_blocks, solid_blocks, ctr;
@@ -1386,7 +1378,7 @@ int batch_prepare(nlos_batch *b, bool &moved) {
         if (int rc = sync_face_buffers(e)) return rc;
         HIP_TRY(hipStreamSynchronize(e->stream));
         for (int ph = 0; ph < 2; ++ph) {
-            StepArgs a = step_args(e, ph, b->block, b->col_planes);
+            StepArgs a = step_args(e, ph, b->block);
             a.step_ctr = b->ctr.as<long long>();
             a.hist = e->hist.as<double>();
             if (!a.hist) a.n_probe_cells = 0;
@@ -1530,7 +1522,6 @@ NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **
     auto bail = [&](int rc) { nlos_batch_destroy(b); return rc; };
     b->mode = cfgs[0].precision;
     if (const char *x = std::getenv("NLOS_BATCH_BLOCK")) parse_block(x, b->block);
-    if (const char *x = std::getenv("NLOS_BATCH_COLUMN")) b->col_planes = std::max(1, std::atoi(x));
     if (const char *x = std::getenv("NLOS_BATCH_RUN")) b->per_group = std::max(1, std::atoi(x));
     if (hipGetDevice(&b->device) != hipSuccess ||
         hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
@@ -1591,29 +1582,23 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
     if (int rc = batch_ready(b)) return rc;
     if (reps < 1) return fail(NLOS_ERR_ARG, "reps must be positive");
     const dim3 blocks[] = {dim3(64, 4, 1), dim3(128, 2, 1), dim3(256, 1, 1)};
-    const int cols[] = {1, 4, 16};
-    const int runs[] = {1, 4, 16};
+    const int runs[] = {1, 2, 4, 8, 16};
     if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
     if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
-    const dim3 keep_block = b->block;
-    const int keep_col = b->col_planes;
     double best = 1e300;
-    dim3 best_block = keep_block;
-    int best_col = keep_col, tried = 0;
-    int best_run = b->per_group;
+    dim3 best_block = b->block;
+    int best_run = b->per_group, tried = 0;
     bool moved = false;
     for (int run : runs)
-    for (const dim3 &bl : blocks)
-        for (int col : cols) {
+        for (const dim3 &bl : blocks) {
             b->block = bl;
-            b->col_planes = col;
             b->per_group = run;
             if (int rc = batch_prepare(b, moved)) return rc;
             // the sweep alone, into the array the next step overwrites: no probe rows, no
             // face records
             std::vector<StepArgs> st;
             for (nlos_engine *e : b->scenes) {
-                StepArgs a = step_args(e, 0, b->block, b->col_planes);
+                StepArgs a = step_args(e, 0, b->block);
                 a.n_probe_cells = 0;
                 a.fb[0] = a.fb[1] = nullptr;
                 st.push_back(a);
@@ -1637,22 +1622,18 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
             if (ms / reps < best) {
                 best = ms / reps;
                 best_block = bl;
-                best_col = col;
                 best_run = b->per_group;
             }
         }
     b->block = best_block;
-    b->col_planes = best_col;
     b->per_group = best_run;
     drop_graph(b);
     b->baked.clear();
     if (result) {
         result->block_x = (int32_t)best_block.x;
         result->block_y = (int32_t)best_block.y;
-        result->col_planes = best_col;
         result->tiles_per_group = best_run;
         result->candidates = tried;
-        result->reserved = 0;
         result->best_ms = best;
     }
     return NLOS_OK;
@@ -1681,7 +1662,7 @@ NLOS_API int nlos_batch_step(nlos_batch *b, int64_t n) {
         HIP_TRY(hipMemcpyAsync(b->ctr.p, b->h_ctr, sizeof b->h_ctr, hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         const std::vector<long long> shape = {
-            (long long)b->block.x, (long long)b->block.y, b->col_planes, (long long)b->per_group, b->total_tiles,
+            (long long)b->block.x, (long long)b->block.y, (long long)b->per_group, b->total_tiles,
             b->total_halo, b->total_solid, b->max_entries};
         if (moved || shape != b->baked) drop_graph(b);
         if (m >= 2 && !b->exec) {
@@ -1865,7 +1846,7 @@ NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_
             GhostArgs ga = ghost_args(e, 0);
             ga.pop = store[k].p;
             ga.fb[0] = ga.fb[1] = nullptr;
-            StepArgs st = step_args(e, 0, e->block, e->col_planes);
+            StepArgs st = step_args(e, 0, e->block);
             st.pop = store[k].p;
             st.pop_out = store[k + 1].p;
             st.n_probe_cells = 0;

@@ -431,41 +431,33 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     }
 }
 
-// Byte-offset access through a wave-uniform base: the address is an SGPR base plus a
-// 32-bit VGPR offset (global_load ... saddr), with no per-lane 64-bit arithmetic.
-// The base goes through readfirstlane: it must be wave-uniform (the same for every active
-// lane), and the barrier keeps the compiler from folding it into a per-lane 64-bit address.
-// (Global address space is kept explicit so that the accesses stay global_load/store.)
+// Wave-uniform value made explicit to the compiler (readfirstlane of both halves).
 __device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// Byte-offset access base + tb.  The address arithmetic is left to the compiler: making
+// each slot's base an opaque wave-uniform value (readfirstlane) kept 38 64-bit bases live
+// in SGPRs, spilled them, and cost more than the per-lane address adds it saved.
 template <class S>
 __device__ __forceinline__ S ld_b(const S *base, unsigned tb) {
-    using GC = __attribute__((address_space(1))) const char;
-    using GS = __attribute__((address_space(1))) const S;
-    GC *b = (GC *)uniform_u64(reinterpret_cast<unsigned long long>(base));
-    return *(GS *)(b + tb);
+    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + tb);
 }
 
 template <class S>
 __device__ __forceinline__ void st_b(S *base, unsigned tb, S v) {
-    using GC = __attribute__((address_space(1))) char;
-    using GS = __attribute__((address_space(1))) S;
-    GC *b = (GC *)uniform_u64(reinterpret_cast<unsigned long long>(base));
-    *(GS *)(b + tb) = v;
+    *reinterpret_cast<S *>(reinterpret_cast<char *>(base) + tb) = v;
 }
 
 // Pull sweep of the two-array pattern (LAY_AB, 05:917-968): f_in_q(c) = cur[q][c - e_q],
 // or the half-way bounce-back cur[INV q][c] when c - e_q is solid; collide; write
 // nxt[q][c] (aligned, local).  Solid lanes store the rest state into their own slots,
 // which nothing reads: every wave then writes whole 128-byte lines, where a line with
-// unwritten lanes costs the memory a read-modify-write.
-//
-// A workgroup walks a column of a.kc planes of its (x, y) tile and issues the loads of
-// plane k+1 before it collides plane k, so that its reads stay in flight while it computes.
+// unwritten lanes costs the memory a read-modify-write.  One thread per cell, one plane
+// per workgroup tile, plain per-lane addressing (the compiler keeps every slot offset as a
+// constant off one cell index).
 
 template <class S>
 __device__ __forceinline__ S read_lane(S v, int src) {
@@ -496,9 +488,7 @@ __device__ __forceinline__ void store_record(S *rec, const C *f, int src, int la
     if (lane < Q) rec[lane] = mine;
 }
 
-// Stored populations and mask byte of one cell (the loads of one plane of a column walk).
-// FP16 values are held two to a register (the loads fill the low and high halves), which
-// halves the registers that the prefetched plane occupies.
+// Stored populations and mask byte of one cell (FP16 values two to a register).
 template <int MODE>
 struct AbIn {
     typename Tr<MODE>::S v[Q];
@@ -517,28 +507,9 @@ struct AbIn<M_F16> {
     }
 };
 
-template <int MODE>
-__device__ __forceinline__ void ab_load(AbIn<MODE> &in, const StepArgs &a, long long cb, unsigned t, unsigned tb) {
-    using S = typename Tr<MODE>::S;
-    const S *cur = static_cast<const S *>(a.pop);
-    const long long N = a.g.ncell;
-    in.m = (a.mask + cb)[t];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const S v = ld_b(cur + ((long long)q * N + cb - nb_off(a.g, q)), tb);
-        if constexpr (MODE == M_F16) {
-            if (q & 1) in.v[q >> 1].y = v;
-            else in.v[q >> 1].x = v;
-        } else {
-            in.v[q] = v;
-        }
-    }
-    if constexpr (MODE == M_F16) in.v[Q >> 1].y = _Float16(0);
-}
-
 template <int MODE, bool CAPTURE>
 __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in, typename Tr<MODE>::C sig,
-                                        int i, int j, int k, long long cb, unsigned t, unsigned tb) {
+                                        int i, int j, int k, long long cb, unsigned t) {
     using C = typename Tr<MODE>::C;
     using S = typename Tr<MODE>::S;
     const long long N = a.g.ncell;
@@ -608,10 +579,10 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
     }
     S *nxt = static_cast<S *>(a.pop_out);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) st_b(nxt + ((long long)q * N + cb), tb, static_cast<S>(f[q]));
+    for (int q = 0; q < Q; ++q) (nxt + ((long long)q * N + cb))[t] = static_cast<S>(f[q]);
 }
 
-// Column b: tile (bi, bj) of the x-y plane, planes k0 .. k0 + a.kc - 1 (clipped to nz).
+// Tile b: (bi, bj) of the x-y plane at plane k.
 template <int MODE, bool CAPTURE>
 __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     using C = typename Tr<MODE>::C;
@@ -619,34 +590,36 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     const int bi = b % a.tiles_x;
     const int rest_b = b / a.tiles_x;
     const int bj = rest_b % a.tiles_y;
-    const int k0 = (rest_b / a.tiles_y) * a.kc + 1;
-    const int k1 = min(k0 + a.kc, a.g.nz + 1);
+    const int k = rest_b / a.tiles_y + 1;
     const int i0 = bi * (int)blockDim.x + 1;
     const int j0 = bj * (int)blockDim.y + 1;
     const int i = i0 + (int)threadIdx.x;
     const int j = j0 + (int)threadIdx.y;
     if (i > a.g.nx || j > a.g.ny) return;
-
-    long long cb = cell_index(a.g, i0, j0, k0);                       // tile corner, uniform
-    const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;  // lane offset (cells)
-    const unsigned tb = t * (unsigned)sizeof(S);                      // lane offset (bytes)
-    const C *sz = static_cast<const C *>(a.sig_z);
-    // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
-    const C sxy = fmax(static_cast<const C *>(a.sig_x)[i], static_cast<const C *>(a.sig_y)[j]);
-
+    const long long N = a.g.ncell;
+    // cell = tile corner (the same for the whole workgroup) + lane offset (32-bit): the
+    // compiler then forms every slot address as a scalar base plus a 32-bit lane offset
+    const long long cb = cell_index(a.g, i0, j0, k);
+    const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;
+    const long long c = cb + t;
+    const S *cur = static_cast<const S *>(a.pop);
     AbIn<MODE> in;
-    ab_load<MODE>(in, a, cb, t, tb);
-    for (int k = k0;; ++k) {
-        if (k + 1 >= k1) {
-            ab_cell<MODE, CAPTURE>(a, in, fmax(sxy, sz[k]), i, j, k, cb, t, tb);
-            break;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const S v = (cur + ((long long)q * N + cb - nb_off(a.g, q)))[t];
+        if constexpr (MODE == M_F16) {
+            if (q & 1) in.v[q >> 1].y = v;
+            else in.v[q >> 1].x = v;
+        } else {
+            in.v[q] = v;
         }
-        AbIn<MODE> nx_in;
-        ab_load<MODE>(nx_in, a, cb + a.g.plane, t, tb);
-        ab_cell<MODE, CAPTURE>(a, in, fmax(sxy, sz[k]), i, j, k, cb, t, tb);
-        in = nx_in;
-        cb += a.g.plane;
     }
+    if constexpr (MODE == M_F16) in.v[Q >> 1].y = _Float16(0);
+    in.m = a.mask[c];
+    // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
+    const C sig = fmax(static_cast<const C *>(a.sig_x)[i],
+                       fmax(static_cast<const C *>(a.sig_y)[j], static_cast<const C *>(a.sig_z)[k]));
+    ab_cell<MODE, CAPTURE>(a, in, sig, i, j, k, cb, t);
 }
 
 template <int MODE, bool CAPTURE>
