@@ -526,14 +526,15 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = in.get(q);
         if (m & MK_NEAR) {
+            // all bounce candidates are loaded together with the wall bits (one memory
+            // round trip), then selected; loads guarded one by one would serialise
             const uint32_t sb = a.wall_bits[c];   // bit q: c - e_q is solid
+            C bb[Q];
+#pragma unroll
+            for (int q = 1; q < Q; ++q) bb[q] = ld<MODE>(a.pop, (long long)INV[q] * N + c, 1.0f);
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
-                if ((sb >> q) & 1u) {
-                    C v = ld<MODE>(a.pop, (long long)INV[q] * N + c, 1.0f);
-                    if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
-                    f[q] = v;
-                }
+                if ((sb >> q) & 1u) f[q] = a.alpha != 1.0 ? reflect<MODE>(q, bb[q], a.alpha) : bb[q];
             }
         }
 
