@@ -245,8 +245,7 @@ def bench_batch(args) -> None:
         e.set_probes(d.probes)
     log(f"batch: {len(defs)} scenes, {cells} cells, set up in {time.perf_counter() - t:.1f} s")
     tune = b.autotune(reps=3)
-    log(f"autotune: {tune.block_x}x{tune.block_y}, tiles per group "
-        f"{tune.tiles_per_group}, {tune.best_ms:.3f} ms "
+    log(f"autotune: {tune.block_x}x{tune.block_y}, {tune.best_ms:.3f} ms "
         f"({tune.candidates} shapes)")
     b.step(2)                      # graph capture outside the timed region
     b.synchronize()
@@ -297,7 +296,7 @@ def bench_batch(args) -> None:
            "config": {"workload": f"{len(defs)} scenes, dims U[64,192]x[64,192]x[64,96], {cells} cells, "
                                   "point sources 2-20 m over reflecting ground (0.6-1.0), half moving "
                                   "1-10 m/s, 4 probes each"},
-           "autotune": {"block": [tune.block_x, tune.block_y], "tiles_per_group": tune.tiles_per_group,
+           "autotune": {"block": [tune.block_x, tune.block_y],
                         "candidates": tune.candidates, "sweep_ms": round(tune.best_ms, 4)},
            "graph_ms_per_step": round(tm.avg_ms, 4),
            "one_scene_at_a_time_mlups": round(seq_mlups, 1),

@@ -1302,8 +1302,8 @@ NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) 
 // Batched scenes.  Every scene is an ordinary engine (two-array pattern); the batch owns
 // them and advances them together with multi-scene launches: per step one source launch
 // (entries of all scenes, indexed by a device step counter), one ghost and one boundary
-// solid launch over the concatenated halo / solid ranges, one sweep whose workgroups each
-// walk a run of consecutive tiles of one scene, and a counter increment.  Kernel arguments of every scene for
+// solid launch over the concatenated halo / solid ranges, one sweep over all scenes' tiles
+// (grid y = scene), and a counter increment.  Kernel arguments of every scene for
 // the current state (phase 0) and the state one step later (phase 1) sit in device arrays,
 // so a captured graph of two steps is replayed for every pair of steps.
 struct nlos_batch {
@@ -1313,10 +1313,10 @@ struct nlos_batch {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     dim3 block{256, 1, 1};
-    int per_group = 4;                    // tiles walked by one sweep workgroup
     DevBuf step_args[2], ghost_args[2], solid_args[2], source_args[2];
     DevBuf tile_table, halo_blocks, solid_blocks, ctr;
-    long long total_tiles = 0, total_halo = 0, total_solid = 0, max_entries = 0;   // table sizes
+    long long total_tiles = 0;            // sweep grid x: the largest scene's tile count
+    long long total_halo = 0, total_solid = 0, max_entries = 0;   // table sizes
     DevBuf sched_off, sched_scene, sched_cells, sched_adds;
     long long h_ctr[2] = {0, 0};          // host staging of the counter and its run base
     hipGraph_t graph = nullptr;
@@ -1371,7 +1371,8 @@ int batch_prepare(nlos_batch *b, bool &moved) {
     std::vector<GhostArgs> gh[2];
     std::vector<SolidArgs> so[2];
     std::vector<SourceArgs> sr[2];
-    std::vector<int2> tiles, halo, solid;   // work tables (scene, tile / first item)
+    std::vector<int> tiles;                // sweep tiles per scene
+    std::vector<int2> halo, solid;         // work tables (scene, first item)
     b->max_entries = 0;
     for (nlos_engine *e : b->scenes) {
         HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1387,8 +1388,8 @@ int batch_prepare(nlos_batch *b, bool &moved) {
             so[ph].push_back(solid_args(e, ph));
             sr[ph].push_back(source_args(e, ph));
         }
-        const int scene = (int)st[0].size() - 1;
-        for (int t = 0; t < st[0].back().nblocks; t += b->per_group) tiles.push_back(make_int2(scene, t));
+        const int scene = (int)tiles.size();
+        tiles.push_back(st[0].back().nblocks);
         for (long long h = 0; h < halo_cells(e->g); h += 256) halo.push_back(make_int2(scene, (int)h));
         for (int q = 0; q < e->n_solid; q += 256) solid.push_back(make_int2(scene, q));
         b->max_entries += stencil_bound(e);
@@ -1399,11 +1400,10 @@ int batch_prepare(nlos_batch *b, bool &moved) {
         if (int rc = upload_vec(b, b->solid_args[ph], so[ph], moved)) return rc;
         if (int rc = upload_vec(b, b->source_args[ph], sr[ph], moved)) return rc;
     }
-    if (tiles.size() > (size_t)INT32_MAX) return fail(NLOS_ERR_UNSUPPORTED, "too many tiles in one batch");
     if (int rc = upload_vec(b, b->tile_table, tiles, moved)) return rc;
     if (int rc = upload_vec(b, b->halo_blocks, halo, moved)) return rc;
     if (int rc = upload_vec(b, b->solid_blocks, solid, moved)) return rc;
-    b->total_tiles = (long long)tiles.size();
+    b->total_tiles = tiles.empty() ? 0 : *std::max_element(tiles.begin(), tiles.end());   // grid x
     b->total_halo = (long long)halo.size();
     b->total_solid = (long long)solid.size();
     (void)n;
@@ -1437,8 +1437,8 @@ int batch_launch_step(nlos_batch *b, int phase) {
             r = LL::solid_multi(b->solid_args[phase].as<SolidArgs>(), b->solid_blocks.as<int2>(),
                                 (int)b->total_solid, b->stream);
         if (r == hipSuccess)
-            r = LL::step_ab_multi(b->step_args[phase].as<StepArgs>(), b->tile_table.as<int2>(),
-                                  (int)b->total_tiles, b->per_group, b->block, b->stream);
+            r = LL::step_ab_multi(b->step_args[phase].as<StepArgs>(), b->tile_table.as<int>(), n,
+                                  (int)b->total_tiles, b->block, b->stream);
         return r;
     }));
     HIP_TRY(launch_count(b->ctr.as<long long>(), 1, b->stream));
@@ -1522,7 +1522,6 @@ NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **
     auto bail = [&](int rc) { nlos_batch_destroy(b); return rc; };
     b->mode = cfgs[0].precision;
     if (const char *x = std::getenv("NLOS_BATCH_BLOCK")) parse_block(x, b->block);
-    if (const char *x = std::getenv("NLOS_BATCH_RUN")) b->per_group = std::max(1, std::atoi(x));
     if (hipGetDevice(&b->device) != hipSuccess ||
         hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
         return bail(fail(NLOS_ERR_HIP, "device query failed"));
@@ -1582,58 +1581,54 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
     if (int rc = batch_ready(b)) return rc;
     if (reps < 1) return fail(NLOS_ERR_ARG, "reps must be positive");
     const dim3 blocks[] = {dim3(64, 4, 1), dim3(128, 2, 1), dim3(256, 1, 1)};
-    const int runs[] = {1, 2, 4, 8, 16};
     if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
     if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
     double best = 1e300;
     dim3 best_block = b->block;
-    int best_run = b->per_group, tried = 0;
+    int tried = 0;
     bool moved = false;
-    for (int run : runs)
-        for (const dim3 &bl : blocks) {
-            b->block = bl;
-            b->per_group = run;
-            if (int rc = batch_prepare(b, moved)) return rc;
-            // the sweep alone, into the array the next step overwrites: no probe rows, no
-            // face records
-            std::vector<StepArgs> st;
-            for (nlos_engine *e : b->scenes) {
-                StepArgs a = step_args(e, 0, b->block);
-                a.n_probe_cells = 0;
-                a.fb[0] = a.fb[1] = nullptr;
-                st.push_back(a);
-            }
-            if (int rc = upload_vec(b, b->step_args[0], st, moved)) return rc;
-            HIP_TRY(hipStreamSynchronize(b->stream));
-            auto sweep = [&]() {
-                return by_mode(b->mode, [&](auto L) {
-                    return decltype(L)::step_ab_multi(b->step_args[0].as<StepArgs>(), b->tile_table.as<int2>(),
-                                                      (int)b->total_tiles, b->per_group, b->block, b->stream);
-                });
-            };
-            HIP_TRY(sweep());
-            HIP_TRY(hipEventRecord(b->ev0, b->stream));
-            for (int r = 0; r < reps; ++r) HIP_TRY(sweep());
-            HIP_TRY(hipEventRecord(b->ev1, b->stream));
-            HIP_TRY(hipEventSynchronize(b->ev1));
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            ++tried;
-            if (ms / reps < best) {
-                best = ms / reps;
-                best_block = bl;
-                best_run = b->per_group;
-            }
+    for (const dim3 &bl : blocks) {
+        b->block = bl;
+        if (int rc = batch_prepare(b, moved)) return rc;
+        // the sweep alone, into the array the next step overwrites: no probe rows, no
+        // face records
+        std::vector<StepArgs> st;
+        for (nlos_engine *e : b->scenes) {
+            StepArgs a = step_args(e, 0, b->block);
+            a.n_probe_cells = 0;
+            a.fb[0] = a.fb[1] = nullptr;
+            st.push_back(a);
         }
+        if (int rc = upload_vec(b, b->step_args[0], st, moved)) return rc;
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        auto sweep = [&]() {
+            return by_mode(b->mode, [&](auto L) {
+                return decltype(L)::step_ab_multi(b->step_args[0].as<StepArgs>(), b->tile_table.as<int>(),
+                                                  (int)b->scenes.size(), (int)b->total_tiles, b->block,
+                                                  b->stream);
+            });
+        };
+        HIP_TRY(sweep());
+        HIP_TRY(hipEventRecord(b->ev0, b->stream));
+        for (int r = 0; r < reps; ++r) HIP_TRY(sweep());
+        HIP_TRY(hipEventRecord(b->ev1, b->stream));
+        HIP_TRY(hipEventSynchronize(b->ev1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        ++tried;
+        if (ms / reps < best) {
+            best = ms / reps;
+            best_block = bl;
+        }
+    }
     b->block = best_block;
-    b->per_group = best_run;
     drop_graph(b);
     b->baked.clear();
     if (result) {
         result->block_x = (int32_t)best_block.x;
         result->block_y = (int32_t)best_block.y;
-        result->tiles_per_group = best_run;
         result->candidates = tried;
+        result->reserved = 0;
         result->best_ms = best;
     }
     return NLOS_OK;
@@ -1662,7 +1657,7 @@ NLOS_API int nlos_batch_step(nlos_batch *b, int64_t n) {
         HIP_TRY(hipMemcpyAsync(b->ctr.p, b->h_ctr, sizeof b->h_ctr, hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         const std::vector<long long> shape = {
-            (long long)b->block.x, (long long)b->block.y, (long long)b->per_group, b->total_tiles,
+            (long long)b->block.x, (long long)b->block.y, b->total_tiles,
             b->total_halo, b->total_solid, b->max_entries};
         if (moved || shape != b->baked) drop_graph(b);
         if (m >= 2 && !b->exec) {

@@ -488,6 +488,13 @@ __device__ __forceinline__ void store_record(S *rec, const C *f, int src, int la
     if (lane < Q) rec[lane] = mine;
 }
 
+// The same pointer in the global address space.  Arguments read from memory (the
+// multi-scene sweep) are generic pointers, which would compile to flat accesses.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *glb(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // Stored populations and mask byte of one cell (FP16 values two to a register).
 template <int MODE>
 struct AbIn {
@@ -528,10 +535,11 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
         if (m & MK_NEAR) {
             // all bounce candidates are loaded together with the wall bits (one memory
             // round trip), then selected; loads guarded one by one would serialise
-            const uint32_t sb = a.wall_bits[c];   // bit q: c - e_q is solid
+            const uint32_t sb = glb(a.wall_bits)[c];   // bit q: c - e_q is solid
+            const auto cur = glb(static_cast<const S *>(a.pop));
             C bb[Q];
 #pragma unroll
-            for (int q = 1; q < Q; ++q) bb[q] = ld<MODE>(a.pop, (long long)INV[q] * N + c, 1.0f);
+            for (int q = 1; q < Q; ++q) bb[q] = static_cast<C>(cur[(long long)INV[q] * N + c]);
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
                 if ((sb >> q) & 1u) f[q] = a.alpha != 1.0 ? reflect<MODE>(q, bb[q], a.alpha) : bb[q];
@@ -578,7 +586,7 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
             for (int q = 0; q < Q; ++q) rec[q] = static_cast<S>(f[q]);
         }
     }
-    S *nxt = static_cast<S *>(a.pop_out);
+    const auto nxt = glb(static_cast<S *>(a.pop_out));
 #pragma unroll
     for (int q = 0; q < Q; ++q) (nxt + ((long long)q * N + cb))[t] = static_cast<S>(f[q]);
 }
@@ -603,7 +611,7 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     const long long cb = cell_index(a.g, i0, j0, k);
     const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;
     const long long c = cb + t;
-    const S *cur = static_cast<const S *>(a.pop);
+    const auto cur = glb(static_cast<const S *>(a.pop));
     AbIn<MODE> in;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -616,10 +624,10 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
         }
     }
     if constexpr (MODE == M_F16) in.v[Q >> 1].y = _Float16(0);
-    in.m = a.mask[c];
+    in.m = glb(a.mask)[c];
     // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
-    const C sig = fmax(static_cast<const C *>(a.sig_x)[i],
-                       fmax(static_cast<const C *>(a.sig_y)[j], static_cast<const C *>(a.sig_z)[k]));
+    const C sig = fmax(glb(static_cast<const C *>(a.sig_x))[i],
+                       fmax(glb(static_cast<const C *>(a.sig_y))[j], glb(static_cast<const C *>(a.sig_z))[k]));
     ab_cell<MODE, CAPTURE>(a, in, sig, i, j, k, cb, t);
 }
 
@@ -630,18 +638,17 @@ __global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
     step_ab_body<MODE, CAPTURE>(a, b);
 }
 
-// Sweep over the tiles of several scenes: workgroup g walks the run of `per_group`
-// consecutive tiles of one scene that runs[g] = (scene, first tile) names, so that the
-// scene's arguments are loaded once per run.
+// Sweep over the tiles of several scenes: workgroup (t, s) sweeps tile t of scene s
+// (blockIdx.y names the scene; workgroups past a scene's tile count return at once).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_step_ab_multi(const StepArgs *__restrict__ scenes,
-                                                       const int2 *__restrict__ runs, int per_group) {
-    const int2 r = runs[blockIdx.x];
+                                                       const int *__restrict__ n_tiles) {
+    const int sc = blockIdx.y;
+    if ((int)blockIdx.x >= n_tiles[sc]) return;
     // a private copy: read through the pointer, every field would be reloaded after each
     // store of the sweep (the compiler cannot rule out aliasing)
-    const StepArgs a = scenes[r.x];
-    const int end = min(r.y + per_group, a.nblocks);
-    for (int b = r.y; b < end; ++b) step_ab_body<MODE, false>(a, b);
+    const StepArgs a = scenes[sc];
+    step_ab_body<MODE, false>(a, blockIdx.x);
 }
 
 // Wall copies of LAY_P (see Layout): for every fluid cell c next to a wall and every q
@@ -1252,10 +1259,10 @@ hipError_t Launch<MODE>::fb_gather(const void *pop, const uint8_t *mask, const G
 }
 
 template <int MODE>
-hipError_t Launch<MODE>::step_ab_multi(const StepArgs *scenes, const int2 *runs, int n_runs, int per_group,
+hipError_t Launch<MODE>::step_ab_multi(const StepArgs *scenes, const int *n_tiles, int n_scenes, int max_tiles,
                                        dim3 block, hipStream_t s) {
-    if (n_runs <= 0) return hipSuccess;
-    k_step_ab_multi<MODE><<<n_runs, block, 0, s>>>(scenes, runs, per_group);
+    if (n_scenes <= 0 || max_tiles <= 0) return hipSuccess;
+    k_step_ab_multi<MODE><<<dim3(max_tiles, n_scenes), block, 0, s>>>(scenes, n_tiles);
     return hipGetLastError();
 }
 

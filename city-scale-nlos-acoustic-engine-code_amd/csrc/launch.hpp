@@ -35,10 +35,10 @@ struct Launch {
     static hipError_t fb_gather(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
                                 void *fb0, void *fb1, float scale, float inv_scale, hipStream_t s);
     // Several scenes of the two-array layout in one launch each.  Argument arrays and the
-    // work tables live on the device: runs[g] = (scene, first tile) of the `per_group`
-    // consecutive tiles that sweep workgroup g walks; blocks[g] = (scene, first item) for
-    // the 256-thread workgroups of the ghost and boundary-solid launches.
-    static hipError_t step_ab_multi(const StepArgs *scenes, const int2 *runs, int n_runs, int per_group,
+    // work tables live on the device: the sweep runs a max_tiles x n_scenes grid (n_tiles
+    // per scene); blocks[g] = (scene, first item) for the 256-thread workgroups of the
+    // ghost and boundary-solid launches.
+    static hipError_t step_ab_multi(const StepArgs *scenes, const int *n_tiles, int n_scenes, int max_tiles,
                                     dim3 block, hipStream_t s);
     static hipError_t ghost_multi(const GhostArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
     static hipError_t solid_multi(const SolidArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);

@@ -75,8 +75,8 @@ class Grad(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("block_x", C.c_int32), ("block_y", C.c_int32), ("tiles_per_group", C.c_int32),
-                ("candidates", C.c_int32), ("best_ms", C.c_double)]
+    _fields_ = [("block_x", C.c_int32), ("block_y", C.c_int32), ("candidates", C.c_int32),
+                ("reserved", C.c_int32), ("best_ms", C.c_double)]
 
 
 # Every C-ABI entry point of include/nlos_engine.h (plus the test hooks of

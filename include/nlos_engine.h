@@ -253,8 +253,8 @@ typedef struct nlos_batch nlos_batch;
 
 typedef struct nlos_tune {
     int32_t block_x, block_y;   /* workgroup shape of the sweep (block_x * block_y = 256) */
-    int32_t tiles_per_group;    /* consecutive tiles of one scene walked by one workgroup */
     int32_t candidates;         /* shapes timed */
+    int32_t reserved;
     double best_ms;             /* sweep time of the chosen shape, all scenes, one step */
 } nlos_tune;
 
@@ -263,7 +263,7 @@ int nlos_batch_destroy(nlos_batch *b);
 int nlos_batch_size(const nlos_batch *b, int32_t *n);
 /* The engine handle of scene i (owned by the batch). */
 int nlos_batch_scene(nlos_batch *b, int32_t i, nlos_engine **scene);
-/* Times the sweep of all scenes for each candidate workgroup shape and tile run (reps
+/* Times the sweep of all scenes for each candidate workgroup shape (reps
  * launches each, into the array the next step overwrites, so the state is untouched) and
  * keeps the fastest.  result may be NULL. */
 int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result);
