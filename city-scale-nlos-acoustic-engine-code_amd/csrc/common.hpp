@@ -87,7 +87,6 @@ struct StepArgs {
     int *flag;                           // bit 0: non-finite density seen
     int tiles_x, tiles_y;                // block tiling of the interior
     int nblocks;
-    int xcd_remap;                       // map consecutive tiles to one XCD
     int dpp;                             // every lane active: odd sweep may shift across lanes
     void *fb[2];                         // x-face buffers (x-, x+) or null, see below
     const uint32_t *wall_bits;           // two-array sweep: bit q set when c - e_q is solid

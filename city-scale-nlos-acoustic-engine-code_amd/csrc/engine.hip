@@ -172,7 +172,6 @@ struct nlos_engine {
     int rec_every = 0;
 
     dim3 block{64, 4, 1};
-    int xcd_remap = 0;
     int use_dpp = 1;
 
     bool timing = false;
@@ -561,7 +560,6 @@ StepArgs step_args(const nlos_engine *e, int phase, dim3 block) {
     a.tiles_x = (e->g.nx + (int)block.x - 1) / (int)block.x;
     a.tiles_y = (e->g.ny + (int)block.y - 1) / (int)block.y;
     a.nblocks = a.tiles_x * a.tiles_y * e->g.nz;
-    a.xcd_remap = e->xcd_remap;
     // lane shifts need full waves: rows of exactly 64-cell tiles, no partial block rows
     a.dpp = e->use_dpp && block.x == 64 && e->g.nx % 64 == 0 && e->g.ny % (int)block.y == 0;
     a.fb[0] = e->fb[0].p;
@@ -749,7 +747,6 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         e->inv_scale = std::ldexp(1.0f, -k);
     }
     const char *block_env = std::getenv("NLOS_BLOCK");
-    if (const char *x = std::getenv("NLOS_XCD_REMAP")) e->xcd_remap = std::atoi(x) != 0;
     if (const char *x = std::getenv("NLOS_DPP")) e->use_dpp = std::atoi(x) != 0;
 
     auto bail = [&](int rc) { nlos_destroy(e); return rc; };

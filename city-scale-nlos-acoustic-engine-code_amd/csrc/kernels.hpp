@@ -191,24 +191,7 @@ __device__ __forceinline__ typename Tr<MODE>::C reflect(int q, typename Tr<MODE>
     else return static_cast<float>(alpha) * v;
 }
 
-// Several scenes in one launch: thread t belongs to the scene s with start[s] <= t < start[s+1].
-__device__ __forceinline__ int scene_of(const long long *start, int n, long long t) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= t) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
 // ---- fused collide-stream sweep ---------------------------------------------------------
-
-__device__ __forceinline__ int remap_block(int b, int nb) {
-    // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of
-    // tiles so that the lines a tile shares with its neighbours sit in the same L2.
-    const int qq = nb >> 3, rr = nb & 7, xcd = b & 7, idx = b >> 3;
-    return xcd < rr ? xcd * (qq + 1) + idx : rr * (qq + 1) + (xcd - rr) * qq + idx;
-}
 
 __device__ __forceinline__ int probe_slot(const long long *cells, int n, long long c) {
     int lo = 0, hi = n - 1;
@@ -221,40 +204,12 @@ __device__ __forceinline__ int probe_slot(const long long *cells, int n, long lo
     return -1;
 }
 
-// Per-lane load/store through a wave-uniform base pointer and a 32-bit lane offset, so
-// that the address is an SGPR base plus a VGPR offset (no per-lane 64-bit arithmetic).
-// Built with -DNLOS_NONTEMPORAL the sweep's streaming accesses carry the non-temporal hint.
-template <class S>
-__device__ __forceinline__ S ld_stream(const S *p) {
-#ifdef NLOS_NONTEMPORAL
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
-}
-
-template <class S>
-__device__ __forceinline__ void st_stream(S *p, S v) {
-#ifdef NLOS_NONTEMPORAL
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
-
-template <class S> __device__ __forceinline__ S ld_b(const S *base, unsigned tb);
-template <class S> __device__ __forceinline__ void st_b(S *base, unsigned tb, S v);
-
-// Lane access base[t] through a wave-uniform base (see ld_b / st_b).  `base` must be the
-// same for every active lane; per-lane addresses use ld / st.
+// Lane access base[t]: the tile base is the same for every lane of the wave, the lane
+// offset t is 32-bit.
 template <int MODE>
 __device__ __forceinline__ typename Tr<MODE>::C ldu(const typename Tr<MODE>::S *base, unsigned t, float inv_scale) {
     using S = typename Tr<MODE>::S;
-#ifdef NLOS_NONTEMPORAL
-    const S v = ld_stream(base + t);
-#else
-    const S v = ld_b(base, t * (unsigned)sizeof(S));
-#endif
+    const S v = base[t];
     if constexpr (MODE == M_F16) return static_cast<float>(v) * inv_scale;
     else return v;
 }
@@ -262,14 +217,8 @@ __device__ __forceinline__ typename Tr<MODE>::C ldu(const typename Tr<MODE>::S *
 template <int MODE>
 __device__ __forceinline__ void stu(typename Tr<MODE>::S *base, unsigned t, typename Tr<MODE>::C v, float scale) {
     using S = typename Tr<MODE>::S;
-    S x;
-    if constexpr (MODE == M_F16) x = static_cast<S>(v * scale);
-    else x = v;
-#ifdef NLOS_NONTEMPORAL
-    st_stream(base + t, x);
-#else
-    st_b(base, t * (unsigned)sizeof(S), x);
-#endif
+    if constexpr (MODE == M_F16) base[t] = static_cast<S>(v * scale);
+    else base[t] = v;
 }
 
 // Whole-wave shift by one lane: with right = true lane L receives lane L-1's value and lane
@@ -302,8 +251,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     using S = typename Tr<MODE>::S;
     // Values stay in stored units through the sweep (scaled FP16 is converted, not rescaled).
     constexpr float RAW = 1.0f;
-    int b = blockIdx.x;
-    if (a.xcd_remap) b = remap_block(b, gridDim.x);
+    const int b = blockIdx.x;
     const int bi = b % a.tiles_x;
     const int rest_b = b / a.tiles_x;
     const int bj = rest_b % a.tiles_y;
@@ -345,7 +293,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
             if (EX[q] == 0) {
                 f[q] = ldu<MODE>(base, t, RAW);
             } else {
-                const S v = ld_stream(base + t);
+                const S v = base[t];
                 const S edge = EX[q] > 0 ? base[row - 1] : base[row + 64];
                 f[q] = (C)wave_shift<S>(v, edge, EX[q] > 0);
             }
@@ -429,26 +377,6 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
                              a.alpha != 1.0 ? reflect<MODE>(q, f[q], a.alpha) : f[q], RAW);
         }
     }
-}
-
-// Wave-uniform value made explicit to the compiler (readfirstlane of both halves).
-__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
-    return ((unsigned long long)hi << 32) | lo;
-}
-
-// Byte-offset access base + tb.  The address arithmetic is left to the compiler: making
-// each slot's base an opaque wave-uniform value (readfirstlane) kept 38 64-bit bases live
-// in SGPRs, spilled them, and cost more than the per-lane address adds it saved.
-template <class S>
-__device__ __forceinline__ S ld_b(const S *base, unsigned tb) {
-    return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + tb);
-}
-
-template <class S>
-__device__ __forceinline__ void st_b(S *base, unsigned tb, S v) {
-    *reinterpret_cast<S *>(reinterpret_cast<char *>(base) + tb) = v;
 }
 
 // Pull sweep of the two-array pattern (LAY_AB, 05:917-968): f_in_q(c) = cur[q][c - e_q],
@@ -633,9 +561,7 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
 
 template <int MODE, bool CAPTURE>
 __global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
-    int b = blockIdx.x;
-    if (a.xcd_remap) b = remap_block(b, gridDim.x);
-    step_ab_body<MODE, CAPTURE>(a, b);
+    step_ab_body<MODE, CAPTURE>(a, blockIdx.x);
 }
 
 // Sweep over the tiles of several scenes: workgroup (t, s) sweeps tile t of scene s
