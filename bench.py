@@ -7,11 +7,13 @@ five faces, a 200 Hz source moving at 20 m/s 30 m above ground with trilinear in
 64 ground-level probes.  A step is one full time step (source, open-face ghosts,
 collide-stream sweep, probe record) over the whole grid.
 
-The headline value is FP16-store (FP32 arithmetic) MLUPS, counted over all cells as in
-04_hpc_scalability.cpp:399-401.  The same run times the FP32 and FP64 paths, reports the
-probe error of FP16/FP32 against FP64 over the same steps, the step kernel's HBM roofline
-fraction (HIP events on the engine stream), and the CPU baseline (the oracle, FP64
-two-array OpenMP restatement of the reference) on a bounded sub-box of the same city.
+The headline value is FP16-store (FP16S: scaled FP16 deviations, FP32 arithmetic) MLUPS,
+counted over all cells as in 04_hpc_scalability.cpp:399-401.  The same run times B16 (16-bit
+mantissas with one exponent byte per cell and an FP32 rest population: the 16-bit storage
+that holds the probe error under 1e-3), FP32 and FP64, reports the probe error of each
+reduced precision against FP64 over the same steps, the headline sweep kernel's HBM
+roofline fraction (HIP events on the engine stream), and the CPU baseline (the oracle,
+FP64 two-array OpenMP restatement of the reference) on a bounded sub-box of the same city.
 """
 from __future__ import annotations
 
@@ -32,15 +34,15 @@ import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per sweep launch (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE,
-# calibrated on a copy kernel of known size) for the default workload; see the file.
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_step_ab_f16s_city1024x1024x256_traffic.md")
+# calibrated on a copy kernel of known size) for the default workload, per storage mode.
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_step_ab_{}_city1024x1024x256_traffic.md")
 METRIC = "MLUPS & % of HBM roofline, city grid: FP16-store vs FP64 at stated probe error"
 
 
-def measured_traffic():
+def measured_traffic(precision):
     """traffic_bytes_per_launch from the committed counter summary, or None."""
     try:
-        with open(TRAFFIC_PROFILE) as fh:
+        with open(TRAFFIC_PROFILE.format(precision)) as fh:
             for line in fh:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -430,8 +432,8 @@ def main() -> None:
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--error-steps", type=int, default=600,
-                    help="steps of the FP16S/FP32 vs FP64 probe-error run (0 to skip)")
-    ap.add_argument("--only", default="f16s,f32,f64")
+                    help="steps of the reduced-precision vs FP64 probe-error run (0 to skip)")
+    ap.add_argument("--only", default="b16,f16s,f32,f64")
     ap.add_argument("--pattern", default="auto", choices=["auto", "inplace", "two_array"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 2 GiB device copy kernel (PMC byte calibration)")
@@ -470,16 +472,16 @@ def main() -> None:
         log(f"  {p}: {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, "
             f"kernel {r['kernel_avg_ms']:.3f} ms ({r['pattern']})")
 
-    head = res.get("f16s") or next(iter(res.values()))
+    head = res.get("f16s") or res.get("b16") or next(iter(res.values()))
     mask = head["mask"]
     fluid = head["cells"] - head["solid"]
     bpu = head["bytes_per_update"]
     alg_bytes = fluid * bpu + head["solid"] * 1.0     # per sweep: 19 loads + 19 stores + mask
     achieved = alg_bytes / (head["kernel_avg_ms"] * 1e-3) / 1e9
-    # The counter summary covers the default workload's FP16S two-array sweep only.
+    # The counter summaries cover the default workload's two-array sweeps only.
     default_wl = (args.nx, args.ny, args.nz) == (1024, 1024, 256)
-    traffic = (measured_traffic() if default_wl and head["precision"] == "f16s"
-               and head["pattern"] == PATTERN_NAMES[2] else None)
+    traffic = (measured_traffic(head["precision"]) if default_wl and head["pattern"] == PATTERN_NAMES[2]
+               else None)
     out = {
         "metric": METRIC,
         "value": round(head["mlups"], 1),
@@ -496,7 +498,9 @@ def main() -> None:
         "data": "synthetic",
         "config": {"workload": f"moving-source city {args.nx}x{args.ny}x{args.nz}, dx=1 m, "
                                f"{len(wl.boxes)} seeded boxes, 64 ground probes, "
-                               f"FP16-scaled deviation storage",
+                               + {"f16s": "FP16-scaled deviation storage",
+                                  "b16": "16-bit block-float deviation storage (B16)"}.get(
+                                      head["precision"], f"{head['precision']} storage"),
                    "cells": head["cells"], "solid_cells": head["solid"],
                    "stream_pattern": head["pattern"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -506,8 +510,9 @@ def main() -> None:
     for p, r in res.items():
         out[f"{p}_mlups"] = round(r["mlups"], 1)
         out[f"{p}_kernel_ms"] = round(r["kernel_avg_ms"], 4)
-    if "f64" in res and "f16s" in res:
-        out["speedup_f16s_vs_f64"] = round(res["f16s"]["mlups"] / res["f64"]["mlups"], 3)
+    for p in ("b16", "f16s", "f32"):
+        if "f64" in res and p in res:
+            out[f"speedup_{p}_vs_f64"] = round(res[p]["mlups"] / res["f64"]["mlups"], 3)
     if args.error_steps > 0 and "f64" in res:
         # Probe error of the reduced-precision paths against FP64 over the same steps
         # (relative L2 over all 64 probe traces, and the worst single probe).

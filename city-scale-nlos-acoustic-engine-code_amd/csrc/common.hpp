@@ -29,8 +29,19 @@ constexpr double CS = 0.57735026918962576451;      // 05:18
 constexpr double PI = 3.14159265358979323846;      // 05:19
 
 // Storage modes.  F64: full populations f, FP64, reference arithmetic order.  F32/F16:
-// deviations g = f - w_q, FP32 arithmetic, stored in FP32 or in FP16 times 2^k.
-enum Mode : int { M_F64 = 0, M_F32 = 1, M_F16 = 2 };
+// deviations g = f - w_q, FP32 arithmetic, stored in FP32 or in FP16 times 2^k.  B16:
+// block floating point per cell in normalised units x_q = g_q / w_q.  The rest slot x_0,
+// which only its own cell reads, is FP32; slots 1..18 hold 16-bit signed mantissas m_q and
+// one signed byte E_c per cell is their shared exponent: x_q = m_q * 2^E_c.  The mantissas
+// of a cell are scaled so that the largest |x_q| fills 15 bits, which keeps 15 significant
+// bits relative to the cell's own amplitude instead of FP16's 11 per value (41 bytes per
+// cell; layout at b16_rest in kernels.hpp).  The exponent ties the 18 slots of a cell
+// together, so B16 runs only with the two-array pattern, where every cell has one writer
+// per step.  B16 mantissas have no NaN or Inf: a non-finite density is reported by the
+// sweep of the step in which it forms (StepArgs::flag).
+enum Mode : int { M_F64 = 0, M_F32 = 1, M_F16 = 2, M_B16 = 3 };
+constexpr int B16_BITS = 15;        // mantissa magnitude bits
+constexpr int B16_EMIN = -113;      // smallest exponent e of max|g/w| < 2^e; E_c = e - 15 >= -128
 
 // Mask byte bits.  The array covers the halo: ground halo cells are SOLID, the other halo
 // cells are fluid (05:509-531).
@@ -96,7 +107,8 @@ struct StepArgs {
 // faces, but in the x-fastest layout their populations sit one per cache line.  The sweep
 // therefore also writes the post-collision populations of those cells, in storage form,
 // to fb[face][(j-1) + ny*(k-1)][q]; the source kernel adds to them like to the lattice, and
-// the ghost kernel reads them contiguously.
+// the ghost kernel reads them contiguously.  B16 records are FP32 deviations g equal to the
+// decoded stored values.
 
 struct GhostArgs {
     void *pop;

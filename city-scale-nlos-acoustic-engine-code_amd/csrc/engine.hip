@@ -197,12 +197,20 @@ struct nlos_engine {
 
 namespace {
 
+// Bytes of one population array: 19 slots per cell; B16 keeps x_0 in FP32, 18 16-bit
+// mantissas and an exponent byte (41 bytes per cell, see Mode in common.hpp).
+size_t pop_bytes(const nlos_engine *e) {
+    if (e->mode == M_B16) return (size_t)e->g.ncell * 41;
+    return (size_t)e->g.ncell * Q * e->s_bytes;
+}
+
 // ---- dispatch over storage modes ------------------------------------------------------
 template <class F>
 hipError_t by_mode(int mode, F &&f) {
     switch (mode) {
     case M_F64: return f(Launch<M_F64>{});
     case M_F32: return f(Launch<M_F32>{});
+    case M_B16: return f(Launch<M_B16>{});
     default: return f(Launch<M_F16>{});
     }
 }
@@ -720,7 +728,9 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     if (!(cfg->dx > 0.0)) return fail(NLOS_ERR_ARG, "dx must be positive");
     if (!(cfg->tau > 0.5)) return fail(NLOS_ERR_ARG, "tau must exceed 0.5 (got %g)", cfg->tau);
     if (!(cfg->c_phys > 0.0)) return fail(NLOS_ERR_ARG, "c_phys must be positive");
-    if (cfg->precision < NLOS_F64 || cfg->precision > NLOS_F16S) return fail(NLOS_ERR_ARG, "bad precision");
+    if (cfg->precision < NLOS_F64 || cfg->precision > NLOS_B16) return fail(NLOS_ERR_ARG, "bad precision");
+    if (cfg->precision == NLOS_B16 && cfg->stream_pattern == NLOS_PATTERN_INPLACE)
+        return fail(NLOS_ERR_UNSUPPORTED, "NLOS_B16 needs the two-array stream pattern");
     if (!(cfg->wall_reflect >= 0.0 && cfg->wall_reflect <= 1.0)) return fail(NLOS_ERR_ARG, "wall_reflect outside [0,1]");
     const int NX = (int)((cfg->x_max - cfg->x_min) / cfg->dx + 0.5);   // 05:332-334
     const int NY = (int)((cfg->y_max - cfg->y_min) / cfg->dx + 0.5);
@@ -739,7 +749,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     e->g.ncell = ((e->g.origin + e->g.plane * e->g.pz + 63) / 64) * 64;
     e->dt = (CS * cfg->dx) / cfg->c_phys;     // 05:200-203
     e->omega = 1.0 / cfg->tau;                // 05:908
-    e->s_bytes = e->mode == M_F64 ? 8 : (e->mode == M_F32 ? 4 : 2);
+    e->s_bytes = e->mode == M_F64 ? 8 : (e->mode == M_F32 ? 4 : 2);   // per population
     e->c_bytes = e->mode == M_F64 ? 8 : 4;
     if (e->mode == M_F16) {
         const int k = cfg->f16_scale_log2 ? cfg->f16_scale_log2 : 24;
@@ -754,7 +764,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(he)));
     he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)));
-    const size_t popb = (size_t)e->g.ncell * Q * e->s_bytes;
+    const size_t popb = pop_bytes(e);
     e->pattern = cfg->stream_pattern;
     if (e->pattern == NLOS_PATTERN_AUTO) {
         size_t freeb = 0, totalb = 0;
@@ -765,6 +775,8 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     }
     if (e->pattern != NLOS_PATTERN_INPLACE && e->pattern != NLOS_PATTERN_TWO_ARRAY)
         return bail(fail(NLOS_ERR_ARG, "bad stream_pattern %d", cfg->stream_pattern));
+    if (e->mode == M_B16 && e->pattern != NLOS_PATTERN_TWO_ARRAY)
+        return bail(fail(NLOS_ERR_NOMEM, "NLOS_B16 needs two population arrays of %.2f GB each", popb / 1e9));
     e->lay = e->pattern == NLOS_PATTERN_TWO_ARRAY ? LAY_AB : LAY_P;
     // Measured on the 1024x1024x256 city: the pull sweep runs fastest on 256-cell row
     // segments, the in-place sweep on 64x4 tiles (its lane shifts need 64-cell rows).
@@ -780,7 +792,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
     for (int f = 0; f < 2; ++f)
         if ((cfg->nscbc_faces >> f) & 1)
-            if (e->fb[f].reserve((size_t)NY * NZ * Q * e->s_bytes) != hipSuccess)
+            if (e->fb[f].reserve((size_t)NY * NZ * Q * (e->mode == M_B16 ? 4 : e->s_bytes)) != hipSuccess)
                 return bail(fail(NLOS_ERR_NOMEM, "cannot allocate x-face buffers"));
     int rc;
     if ((rc = upload_sponge(e)) != NLOS_OK) return bail(rc);
@@ -834,7 +846,7 @@ NLOS_API int nlos_get_info(const nlos_engine *e, nlos_info *info) {
     info->step = e->step;
     info->n_probes = (int64_t)e->probes.size();
     info->n_solid_boundary = e->n_solid;
-    info->bytes_per_update = 2.0 * Q * (double)e->s_bytes + 1.0;
+    info->bytes_per_update = e->mode == M_B16 ? 2.0 * 41 + 1.0 : 2.0 * Q * (double)e->s_bytes + 1.0;
     size_t tot = 0;
     const DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                             &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
@@ -1154,7 +1166,7 @@ NLOS_API int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *need
     h.nx = e->g.nx; h.ny = e->g.ny; h.nz = e->g.nz; h.lay = e->lay;
     h.step = e->step; h.ncell = e->g.ncell; h.n_solid = e->n_solid;
     h.n_ucells = (int64_t)e->probe_ucells.size();
-    h.pop_bytes = (uint64_t)e->g.ncell * Q * e->s_bytes;
+    h.pop_bytes = pop_bytes(e);
     h.state_bytes = (uint64_t)e->n_solid * Q * e->c_bytes;
     h.hist_bytes = (uint64_t)e->step * h.n_ucells * 8;
     const size_t total = sizeof h + h.pop_bytes + h.state_bytes + h.hist_bytes;
@@ -1260,6 +1272,42 @@ NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, in
     } else if (e->mode == M_F32) {
         const float v = (float)(f - W64[q]);
         HIP_TRY(hipMemcpy(static_cast<float *>(e->curp()) + idx, &v, 4, hipMemcpyHostToDevice));
+    } else if (e->mode == M_B16) {
+        // re-code the whole cell with the new value (see Mode in common.hpp)
+        const long long N = e->g.ncell, c = e->cell(i + 1, j + 1, k + 1);
+        char *base = static_cast<char *>(e->curp());
+        float *rp = reinterpret_cast<float *>(base) + c;
+        short *mp = reinterpret_cast<short *>(base) + N;
+        signed char *ep = reinterpret_cast<signed char *>(base) + 40 * N + c;
+        short m[Q] = {0};
+        signed char E;
+        float x0;
+        for (int r = 1; r < Q; ++r) HIP_TRY(hipMemcpy(&m[r], mp + (size_t)r * N + c, 2, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&E, ep, 1, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&x0, rp, 4, hipMemcpyDeviceToHost));
+        double x[Q], mx = 0.0;
+        x[0] = x0;
+        for (int r = 1; r < Q; ++r) x[r] = std::ldexp((double)m[r], E);
+        x[q] = (f - W64[q]) / W64[q];
+        for (int r = 1; r < Q; ++r) mx = std::max(mx, std::fabs(x[r]));
+        int ex = 0;
+        std::frexp((float)mx, &ex);
+        ex = std::max(ex, B16_EMIN);
+        for (int r = 1; r < Q; ++r) {
+            const double v = std::nearbyint(std::ldexp(x[r], B16_BITS - ex));
+            m[r] = (short)std::min(std::max(v, -32768.0), 32767.0);
+        }
+        E = (signed char)(ex - B16_BITS);
+        x0 = (float)x[0];
+        if (!std::isfinite(f) && q > 0) {
+            // B16 mantissas have no NaN or Inf; the largest exponent decodes 32767 * 2^127 to
+            // +Inf in FP32, so the next sweep forms a non-finite density as the other modes do
+            m[q] = 32767;
+            E = 127;
+        }
+        for (int r = 1; r < Q; ++r) HIP_TRY(hipMemcpy(mp + (size_t)r * N + c, &m[r], 2, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ep, &E, 1, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(rp, &x0, 4, hipMemcpyHostToDevice));
     } else {
         const _Float16 v = (_Float16)((float)(f - W64[q]) * e->scale);
         HIP_TRY(hipMemcpy(static_cast<_Float16 *>(e->curp()) + idx, &v, 2, hipMemcpyHostToDevice));
@@ -1723,12 +1771,11 @@ int adjoint_supported(const nlos_engine *e) {
     if (e->pattern != NLOS_PATTERN_TWO_ARRAY)
         return fail(NLOS_ERR_UNSUPPORTED, "the adjoint needs the two-array stream pattern");
     if (e->cfg.nscbc_faces != 0) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint needs nscbc_faces == 0");
-    if (e->mode == M_F16) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint supports NLOS_F64 and NLOS_F32");
+    if (e->mode == M_F16 || e->mode == M_B16)
+        return fail(NLOS_ERR_UNSUPPORTED, "the adjoint supports NLOS_F64 and NLOS_F32");
     if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch");
     return NLOS_OK;
 }
-
-size_t pop_bytes(const nlos_engine *e) { return (size_t)e->g.ncell * Q * e->s_bytes; }
 
 }  // namespace
 

@@ -24,6 +24,7 @@ template <int MODE> struct Tr;
 template <> struct Tr<M_F64> { using S = double; using C = double; };
 template <> struct Tr<M_F32> { using S = float; using C = float; };
 template <> struct Tr<M_F16> { using S = _Float16; using C = float; };
+template <> struct Tr<M_B16> { using S = short; using C = float; };   // see Mode
 
 template <int MODE>
 __device__ __forceinline__ typename Tr<MODE>::C ld(const void *base, long long idx, float inv_scale) {
@@ -49,6 +50,84 @@ __device__ __forceinline__ typename Tr<MODE>::C rest(int q) {
 
 template <class C>
 __device__ __forceinline__ C wq(int q) { return static_cast<C>(W64[q]); }
+
+// ---- B16 block floating point (see Mode) --------------------------------------------------
+// Cells are coded in normalised units x_q = g_q / w_q.  x_0 (the rest slot, read only by its
+// own cell) is kept in FP32.  The exponent e of a cell is the smallest with
+// max_{q>=1} |x_q| < 2^e; the mantissas are x_q * 2^(15 - e) rounded to nearest even and
+// saturated to 16 bits, and the stored byte is E = e - 15, so that x_q = m_q * 2^E.
+// Array of N = ncell cells: bytes [0, 4N) x_0 (FP32), [4N, 40N) mantissas (slot q >= 1 at
+// 16-bit index N + q*N + c), [40N, 41N) exponents.
+
+__device__ __forceinline__ const float *b16_rest(const void *pop) { return static_cast<const float *>(pop); }
+__device__ __forceinline__ float *b16_rest(void *pop) { return static_cast<float *>(pop); }
+__device__ __forceinline__ const short *b16_mants(const void *pop, long long N) {
+    return static_cast<const short *>(pop) + N;
+}
+__device__ __forceinline__ short *b16_mants(void *pop, long long N) { return static_cast<short *>(pop) + N; }
+__device__ __forceinline__ const signed char *b16_exps(const void *pop, long long N) {
+    return static_cast<const signed char *>(pop) + 40 * N;
+}
+__device__ __forceinline__ signed char *b16_exps(void *pop, long long N) {
+    return static_cast<signed char *>(pop) + 40 * N;
+}
+__device__ __forceinline__ float b16_dec(int m, int E) { return __builtin_amdgcn_ldexpf((float)m, E); }
+
+// Exponent e of a cell (clamped below so that e - 15 fits a signed byte).
+__device__ __forceinline__ int b16_exp_of(const float *x) {
+    float mx = 0.0f;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) mx = fmaxf(mx, fabsf(x[q]));
+    return max(__builtin_amdgcn_frexp_expf(mx), B16_EMIN);   // mx = frac * 2^e, frac in [0.5, 1)
+}
+
+// Mantissas of slots 1..18 at exponent e, two per saturating pack (|x| < 2^e rounds to at
+// most 32768, which saturates to 32767).
+__device__ __forceinline__ void b16_code(const float *x, int e, short *m) {
+    const float sc = __builtin_amdgcn_ldexpf(1.0f, B16_BITS - e);
+    m[0] = 0;
+#pragma unroll
+    for (int q = 1; q < Q; q += 2) {
+        const int a = (int)__builtin_rintf(x[q] * sc);
+        const int b = q + 1 < Q ? (int)__builtin_rintf(x[q + 1] * sc) : 0;
+        const auto p = __builtin_amdgcn_cvt_pk_i16(a, b);
+        m[q] = p.x;
+        if (q + 1 < Q) m[q + 1] = p.y;
+    }
+}
+
+// Whole-cell access in deviation units g (the kernels other than the sweep).
+__device__ __forceinline__ void b16_load_g(const void *pop, long long N, long long c, float *g) {
+    const short *m = b16_mants(pop, N);
+    const int E = b16_exps(pop, N)[c];
+    g[0] = b16_rest(pop)[c] * wq<float>(0);
+#pragma unroll
+    for (int q = 1; q < Q; ++q) g[q] = b16_dec(m[(long long)q * N + c], E) * wq<float>(q);
+}
+__device__ __forceinline__ float b16_slot_g(const void *pop, long long N, long long c, int q) {
+    if (q == 0) return b16_rest(pop)[c] * wq<float>(0);
+    return b16_dec(b16_mants(pop, N)[(long long)q * N + c], b16_exps(pop, N)[c]) * wq<float>(q);
+}
+// Codes x into cell c; slots with write[q] false are left as they are (the ghost layer
+// writes only the slots that interior cells pull).  Returns e.
+__device__ __forceinline__ int b16_store_x(void *pop, long long N, long long c, const float *x,
+                                           unsigned write = 0xffffffffu) {
+    const int e = b16_exp_of(x);
+    short m[Q];
+    b16_code(x, e, m);
+    if (write & 1u) b16_rest(pop)[c] = x[0];
+    short *mp = b16_mants(pop, N);
+#pragma unroll
+    for (int q = 1; q < Q; ++q)
+        if ((write >> q) & 1u) mp[(long long)q * N + c] = m[q];
+    b16_exps(pop, N)[c] = (signed char)(e - B16_BITS);
+    return e;
+}
+// g -> x: 1/w_q = 3, 18, 36 are exact in FP32.
+__device__ __forceinline__ void b16_g_to_x(const float *g, float *x) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) x[q] = g[q] * (q == 0 ? 3.0f : (q < 7 ? 18.0f : 36.0f));
+}
 
 // Neighbour offset of direction q.
 __device__ __forceinline__ long long nb_off(const GridDesc &g, int q) {
@@ -179,6 +258,45 @@ __device__ __forceinline__ float collide32(float *G, float omega, float keep, fl
         const float odd = c3 * cu;
         G[q] = a * G[q] + (even + odd);
         G[p] = a * G[p] + (even - odd);
+    }
+    return dr;
+}
+
+// BGK collision of collide32 on normalised deviations x_q = g_q / w_q (B16 storage): the
+// weights move into the moment sums and drop out of the update,
+//   x*_q = keep*((1-omega)*x_q + omega*(drho + rho*(3cu + 4.5cu^2 - 1.5u^2))).
+// Returns drho.
+__device__ __forceinline__ float collide_n(float *X, float omega, float keep) {
+    constexpr int PAIR[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+    float sp[9], dp[9];
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+        sp[n] = X[PAIR[n]] + X[INV[PAIR[n]]];
+        dp[n] = X[PAIR[n]] - X[INV[PAIR[n]]];
+    }
+    constexpr float W0 = (float)(1.0 / 3.0), W1 = (float)(1.0 / 18.0), W2 = (float)(1.0 / 36.0);
+    float dr = X[0] * W0 + (sp[0] + sp[1] + sp[2]) * W1 +
+               (((sp[3] + sp[4]) + (sp[5] + sp[6])) + (sp[7] + sp[8])) * W2;
+    const float jx = dp[0] * W1 + (((dp[3] + dp[4]) + dp[5]) + dp[6]) * W2;
+    const float jy = dp[1] * W1 + ((dp[3] - dp[4]) + (dp[7] + dp[8])) * W2;
+    const float jz = dp[2] * W1 + ((dp[5] - dp[6]) + (dp[7] - dp[8])) * W2;
+    float r = 1.0f + dr;
+    if (r <= 0.0f) { r = 1e-12f; dr = r - 1.0f; }
+    const float ir = __builtin_amdgcn_rcpf(r);
+    const float ux = jx * ir, uy = jy * ir, uz = jz * ir;
+    const float u2c = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float a = keep * (1.0f - omega);
+    const float bw = keep * omega;
+    const float c1 = bw * dr, c2 = bw * r, c3 = 3.0f * c2;
+    X[0] = a * X[0] + (c1 - c2 * u2c);
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+        const int q = PAIR[n], p = INV[q];
+        const float cu = cu_of<float>(q, ux, uy, uz);
+        const float even = c2 * ((4.5f * cu) * cu - u2c) + c1;
+        const float odd = c3 * cu;
+        X[q] = a * X[q] + (even + odd);
+        X[p] = a * X[p] + (even - odd);
     }
     return dr;
 }
@@ -442,6 +560,17 @@ struct AbIn<M_F16> {
     }
 };
 
+// B16: the cell's own x_0, and for q >= 1 the mantissa of slot q of x - e_q and that cell's
+// exponent (e[0] is the cell's own); get() is the normalised value x_q.
+template <>
+struct AbIn<M_B16> {
+    float r0;
+    short v[Q];
+    signed char e[Q];
+    uint8_t m;
+    __device__ __forceinline__ float get(int q) const { return q == 0 ? r0 : b16_dec(v[q], e[q]); }
+};
+
 template <int MODE, bool CAPTURE>
 __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in, typename Tr<MODE>::C sig,
                                         int i, int j, int k, long long cb, unsigned t) {
@@ -464,10 +593,14 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
             // all bounce candidates are loaded together with the wall bits (one memory
             // round trip), then selected; loads guarded one by one would serialise
             const uint32_t sb = glb(a.wall_bits)[c];   // bit q: c - e_q is solid
-            const auto cur = glb(static_cast<const S *>(a.pop));
+            const auto cur = glb(static_cast<const S *>(MODE == M_B16 ? (const void *)b16_mants(a.pop, N) : a.pop));
             C bb[Q];
 #pragma unroll
-            for (int q = 1; q < Q; ++q) bb[q] = static_cast<C>(cur[(long long)INV[q] * N + c]);
+            for (int q = 1; q < Q; ++q) {
+                // B16: slot INV[q] of the cell itself, in its own exponent (w_INV[q] = w_q)
+                if constexpr (MODE == M_B16) bb[q] = b16_dec(cur[(long long)INV[q] * N + c], in.e[0]);
+                else bb[q] = static_cast<C>(cur[(long long)INV[q] * N + c]);
+            }
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
                 if ((sb >> q) & 1u) f[q] = a.alpha != 1.0 ? reflect<MODE>(q, bb[q], a.alpha) : bb[q];
@@ -480,6 +613,10 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
             const double r = collide64(f, a.omega, sig);
             dval = r - 1.0;
             finite = isfinite(r);
+        } else if constexpr (MODE == M_B16) {
+            const float dr = collide_n(f, (float)a.omega, 1.0f - sig);
+            dval = dr;
+            finite = isfinite(dr);
         } else {
             const float dr = collide32(f, (float)a.omega, 1.0f - sig, a.scale, a.inv_scale);
             dval = dr;
@@ -496,6 +633,13 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
                 (float)dval;
         }
     }
+    // B16: code the cell first, so that the x-face records carry the stored values
+    [[maybe_unused]] int be = 0;
+    [[maybe_unused]] short bm[Q];
+    if constexpr (MODE == M_B16) {
+        be = b16_exp_of(f);
+        b16_code(f, be, bm);
+    }
     // x-face records of the cells (1, j, k) and (nx, j, k) when fluid (see StepArgs::fb);
     // every test below is uniform over the wave (one row segment of 64 cells).
     const int lane = (int)(threadIdx.x & 63u);
@@ -506,17 +650,33 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
         if (face == 0 ? iw != 1 : !(iw <= a.g.nx && a.g.nx < iw + 64)) continue;
         const int src = face == 0 ? 0 : a.g.nx - iw;
         if (__builtin_amdgcn_readlane((int)m, src) & MK_SOLID) continue;
-        S *rec = static_cast<S *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
+        // B16 records hold the stored values decoded to FP32 deviations g (see StepArgs::fb)
+        using R = std::conditional_t<MODE == M_B16, float, S>;
+        C fr[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            if constexpr (MODE == M_B16) fr[q] = (q == 0 ? f[0] : b16_dec(bm[q], be - B16_BITS)) * wq<float>(q);
+            else fr[q] = f[q];
+        }
+        R *rec = static_cast<R *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
         if (a.g.nx - iw >= Q - 1) {
-            store_record(rec, f, src, lane);
+            store_record(rec, fr, src, lane);
         } else if (lane == src) {   // fewer than Q active lanes in this row segment
 #pragma unroll
-            for (int q = 0; q < Q; ++q) rec[q] = static_cast<S>(f[q]);
+            for (int q = 0; q < Q; ++q) rec[q] = static_cast<R>(fr[q]);
         }
     }
     const auto nxt = glb(static_cast<S *>(a.pop_out));
+    if constexpr (MODE == M_B16) {
+        (glb(b16_rest(a.pop_out)) + cb)[t] = f[0];
+        const auto nm = glb(b16_mants(a.pop_out, N));
 #pragma unroll
-    for (int q = 0; q < Q; ++q) (nxt + ((long long)q * N + cb))[t] = static_cast<S>(f[q]);
+        for (int q = 1; q < Q; ++q) (nm + ((long long)q * N + cb))[t] = bm[q];
+        (glb(b16_exps(a.pop_out, N)) + cb)[t] = (signed char)(be - B16_BITS);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) (nxt + ((long long)q * N + cb))[t] = static_cast<S>(f[q]);
+    }
 }
 
 // Tile b: (bi, bj) of the x-y plane at plane k.
@@ -539,19 +699,30 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     const long long cb = cell_index(a.g, i0, j0, k);
     const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;
     const long long c = cb + t;
-    const auto cur = glb(static_cast<const S *>(a.pop));
     AbIn<MODE> in;
+    if constexpr (MODE == M_B16) {
+        const auto ex = glb(b16_exps(a.pop, N));
+        const auto mn = glb(b16_mants(a.pop, N));
+        in.r0 = (glb(b16_rest(a.pop)) + cb)[t];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const S v = (cur + ((long long)q * N + cb - nb_off(a.g, q)))[t];
-        if constexpr (MODE == M_F16) {
-            if (q & 1) in.v[q >> 1].y = v;
-            else in.v[q >> 1].x = v;
-        } else {
-            in.v[q] = v;
+        for (int q = 0; q < Q; ++q) in.e[q] = (ex + (cb - nb_off(a.g, q)))[t];
+#pragma unroll
+        for (int q = 1; q < Q; ++q) in.v[q] = (mn + ((long long)q * N + cb - nb_off(a.g, q)))[t];
+        in.v[0] = 0;
+    } else {
+        const auto cur = glb(static_cast<const S *>(a.pop));
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const S v = (cur + ((long long)q * N + cb - nb_off(a.g, q)))[t];
+            if constexpr (MODE == M_F16) {
+                if (q & 1) in.v[q >> 1].y = v;
+                else in.v[q >> 1].x = v;
+            } else {
+                in.v[q] = v;
+            }
         }
+        if constexpr (MODE == M_F16) in.v[Q >> 1].y = _Float16(0);
     }
-    if constexpr (MODE == M_F16) in.v[Q >> 1].y = _Float16(0);
     in.m = glb(a.mask)[c];
     // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
     const C sig = fmax(glb(static_cast<const C *>(a.sig_x))[i],
@@ -726,7 +897,10 @@ __device__ __forceinline__ typename Tr<MODE>::C read_post(const void *pop, const
                                                           const GridDesc &g, long long c, int q,
                                                           float inv_scale) {
     const long long N = g.ncell;
-    if constexpr (LAY == LAY_AB) {
+    if constexpr (MODE == M_B16) {
+        static_assert(LAY == LAY_AB, "B16 runs with the two-array pattern only");
+        return b16_slot_g(pop, N, c, q);
+    } else if constexpr (LAY == LAY_AB) {
         return ld<MODE>(pop, (long long)q * N + c, inv_scale);
     } else if constexpr (LAY == LAY_P) {
         return ld<MODE>(pop, (long long)INV[q] * N + c, inv_scale);
@@ -791,10 +965,16 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
         const int ii = i - oi, jj = j - oj, kk = k - ok;
         const long long I = cell_index(g, ii, jj, kk);
         if (axis == 0 && a.fb[face & 1] && !(a.mask[I] & MK_SOLID)) {
-            const typename Tr<MODE>::S *fb = static_cast<const typename Tr<MODE>::S *>(a.fb[face & 1]) +
-                                             ((long long)(jj - 1) + (long long)ny * (kk - 1)) * Q;
+            const long long r0 = ((long long)(jj - 1) + (long long)ny * (kk - 1)) * Q;
+            if constexpr (MODE == M_B16) {
+                const float *fb = static_cast<const float *>(a.fb[face & 1]) + r0;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) fI[q] = ld<MODE>(fb, q, a.inv_scale);   // per-thread address
+                for (int q = 0; q < Q; ++q) fI[q] = fb[q];
+            } else {
+                const typename Tr<MODE>::S *fb = static_cast<const typename Tr<MODE>::S *>(a.fb[face & 1]) + r0;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) fI[q] = ld<MODE>(fb, q, a.inv_scale);   // per-thread address
+            }
         } else if (a.mask[I] & MK_SOLID) {
             int pidx;
             if (axis == 0) pidx = (jj - 1) + ny * (kk - 1);
@@ -804,6 +984,8 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
             const C *so = static_cast<const C *>(a.s_old);
 #pragma unroll
             for (int q = 0; q < Q; ++q) fI[q] = so[(long long)slot * Q + q];
+        } else if constexpr (MODE == M_B16) {
+            b16_load_g(a.pop, g.ncell, I, fI);
         } else if (LAY != LAY_Q || !(a.mask[I] & MK_NEAR)) {
             // f*_q(I) in slot q of I (LAY_AB), slot INV[q] of I (LAY_P), or, with no solid
             // neighbour, slot q of I + e_q (LAY_Q)
@@ -859,6 +1041,10 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
         }
     }
 
+    C gout[Q];            // B16: the cell is coded once, after the loop
+    unsigned pulled = 0;  // B16: slots that interior cells pull
+#pragma unroll
+    for (int q = 0; q < Q; ++q) gout[q] = rest<MODE>(q);
 #pragma unroll
     for (int q = 1; q < Q; ++q) {
         const int ti = i + EX[q], tj = j + EY[q], tk = k + EZ[q];
@@ -876,16 +1062,28 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
                 val = geB + (fI[q] - geI);
             }
         }
-        const long long tc = h + nb_off(g, q);
-        long long idx;
-        if constexpr (LAY == LAY_AB) {
-            idx = (long long)q * N + h;
-        } else if constexpr (LAY == LAY_P) {
-            idx = (long long)INV[q] * N + h;
+        if constexpr (MODE == M_B16) {
+            gout[q] = val;
+            pulled |= 1u << q;
+            continue;
         } else {
-            idx = (a.mask[tc] & MK_SOLID) ? (long long)INV[q] * N + h : (long long)q * N + tc;
+            const long long tc = h + nb_off(g, q);
+            long long idx;
+            if constexpr (LAY == LAY_AB) {
+                idx = (long long)q * N + h;
+            } else if constexpr (LAY == LAY_P) {
+                idx = (long long)INV[q] * N + h;
+            } else {
+                idx = (a.mask[tc] & MK_SOLID) ? (long long)INV[q] * N + h : (long long)q * N + tc;
+            }
+            st<MODE>(a.pop, idx, val, a.scale);
         }
-        st<MODE>(a.pop, idx, val, a.scale);
+    }
+    if constexpr (MODE == M_B16) {
+        // slots no interior cell pulls are never read: only the pulled ones are written
+        float x[Q];
+        b16_g_to_x(gout, x);
+        b16_store_x(a.pop, N, h, x, pulled);
     }
 }
 
@@ -924,6 +1122,8 @@ __device__ __forceinline__ void solid_body(const SolidArgs &a, int n) {
         if (a.mask[src] & MK_SOLID) {
             v = so[(long long)n * Q + INV[q]];
             if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
+        } else if constexpr (MODE == M_B16) {
+            v = b16_slot_g(a.pop, N, src, q);
         } else {
             v = ld<MODE>(a.pop, (long long)(a.lay == LAY_AB ? q : INV[q]) * N + src, a.inv_scale);
         }
@@ -992,11 +1192,43 @@ __device__ __forceinline__ void source_body(const SourceArgs &a, int e, int q) {
     }
 }
 
+// B16 (two-array pattern): one thread per stencil cell adds to all 19 slots and re-codes
+// the cell; g_q += w_q * add is x_q += add in normalised units.
+__device__ __forceinline__ void source_cell_b16(const SourceArgs &a, int e) {
+    const long long c = a.cells[e];
+    const long long N = a.g.ncell;
+    const float add = (float)a.adds[e];
+    float x[Q];
+    b16_load_g(a.pop, N, c, x);
+    b16_g_to_x(x, x);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) x[q] += add;
+    const int ex = b16_store_x(a.pop, N, c, x);
+    // an x-face record of this cell takes the stored values (as the sweep writes them)
+    const long long r = c - a.g.origin;
+    const int i = (int)(r % a.g.px);
+    for (int face = 0; face < 2; ++face) {
+        if ((face == 0 ? i != 1 : i != a.g.nx) || !a.fb[face]) continue;
+        const int j = (int)((r / a.g.px) % a.g.py);
+        const int k = (int)(r / a.g.plane);
+        float *rec = static_cast<float *>(a.fb[face]) + ((long long)(j - 1) + (long long)a.g.ny * (k - 1)) * Q;
+        short m[Q];
+        b16_code(x, ex, m);
+        rec[0] = x[0] * wq<float>(0);
+#pragma unroll
+        for (int q = 1; q < Q; ++q) rec[q] = b16_dec(m[q], ex - B16_BITS) * wq<float>(q);
+    }
+}
+
 template <int MODE, int LAY>
 __global__ __launch_bounds__(256) void k_source(SourceArgs a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n * Q) return;
-    source_body<MODE, LAY>(a, t / Q, t % Q);
+    if constexpr (MODE == M_B16) {
+        if (t < a.n) source_cell_b16(a, t);
+    } else {
+        if (t >= a.n * Q) return;
+        source_body<MODE, LAY>(a, t / Q, t % Q);
+    }
 }
 
 // All scenes' source entries of the current step (index ctr - ctr_base into step_off).
@@ -1005,13 +1237,15 @@ __global__ __launch_bounds__(256) void k_source_multi(SourceMulti m) {
     const long long st = *m.ctr - *m.ctr_base;
     const long long e0 = m.step_off[st], e1 = m.step_off[st + 1];
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (e1 - e0) * Q) return;
-    const long long e = e0 + t / Q;
+    constexpr int PER = MODE == M_B16 ? 1 : Q;   // threads per entry
+    if (t >= (e1 - e0) * PER) return;
+    const long long e = e0 + t / PER;
     SourceArgs a = m.scenes[m.entry_scene[e]];
     a.cells = m.cells + e;
     a.adds = m.adds + e;
     a.n = 1;
-    source_body<MODE, LAY>(a, 0, (int)(t % Q));
+    if constexpr (MODE == M_B16) source_cell_b16(a, 0);
+    else source_body<MODE, LAY>(a, 0, (int)(t % Q));
 }
 
 // Fills the x-face buffers from the lattice (after the state was set from outside).
@@ -1029,8 +1263,11 @@ __global__ __launch_bounds__(256) void k_fb_gather(const void *pop, const uint8_
     const long long I = cell_index(g, face == 0 ? 1 : g.nx, j, k);
     if (mask[I] & MK_SOLID) return;
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
-        st<MODE>(fb, jk * Q + q, read_post<MODE, LAY>(pop, mask, g, I, q, inv_scale), scale);
+    for (int q = 0; q < Q; ++q) {
+        const auto v = read_post<MODE, LAY>(pop, mask, g, I, q, inv_scale);
+        if constexpr (MODE == M_B16) static_cast<float *>(fb)[jk * Q + q] = v;
+        else st<MODE>(fb, jk * Q + q, v, scale);
+    }
 }
 
 // ---- initialisation and gathers ------------------------------------------------------------
@@ -1038,8 +1275,16 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_fill_rest(void *pop, long long ncell) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncell) return;
+    if constexpr (MODE == M_B16) {
+        b16_rest(pop)[t] = 0.0f;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) st<MODE>(pop, (long long)q * ncell + t, rest<MODE>(q), 1.0f);
+        for (int q = 1; q < Q; ++q) b16_mants(pop, ncell)[(long long)q * ncell + t] = 0;
+        b16_exps(pop, ncell)[t] = 0;
+        return;
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) st<MODE>(pop, (long long)q * ncell + t, rest<MODE>(q), 1.0f);
+    }
 }
 
 template <int MODE>
@@ -1054,12 +1299,19 @@ __global__ __launch_bounds__(256) void k_set_density(void *pop, const uint8_t *m
     const long long c = cell_index(g, i, j, k);
     if (mask[c] & MK_SOLID) return;
     const double r = rho[n];
+    if constexpr (MODE == M_B16) {
+        float x[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) x[q] = (float)(r - 1.0);   // g = w (rho - 1)
+        b16_store_x(pop, g.ncell, c, x);
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         // feq(q, rho, 0, 0, 0) = w_q * rho (05:286-291 with u = 0); symmetric in q, so the
         // value is the same in either slot convention.
         if constexpr (MODE == M_F64) st<MODE>(pop, (long long)q * g.ncell + c, W64[q] * r, scale);
-        else st<MODE>(pop, (long long)q * g.ncell + c, (float)(W64[q] * (r - 1.0)), scale);
+        else if constexpr (MODE != M_B16) st<MODE>(pop, (long long)q * g.ncell + c, (float)(W64[q] * (r - 1.0)), scale);
     }
 }
 
@@ -1098,15 +1350,19 @@ inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1)
 
 template <int MODE>
 hipError_t Launch<MODE>::step(const StepArgs &a, bool odd, bool capture, dim3 block, hipStream_t s) {
-    const dim3 grid(a.nblocks);
-    if (odd) {
-        if (capture) k_step<MODE, true, true><<<grid, block, 0, s>>>(a);
-        else k_step<MODE, true, false><<<grid, block, 0, s>>>(a);
+    if constexpr (MODE == M_B16) {
+        return hipErrorNotSupported;   // two-array pattern only
     } else {
-        if (capture) k_step<MODE, false, true><<<grid, block, 0, s>>>(a);
-        else k_step<MODE, false, false><<<grid, block, 0, s>>>(a);
+        const dim3 grid(a.nblocks);
+        if (odd) {
+            if (capture) k_step<MODE, true, true><<<grid, block, 0, s>>>(a);
+            else k_step<MODE, true, false><<<grid, block, 0, s>>>(a);
+        } else {
+            if (capture) k_step<MODE, false, true><<<grid, block, 0, s>>>(a);
+            else k_step<MODE, false, false><<<grid, block, 0, s>>>(a);
+        }
+        return hipGetLastError();
     }
-    return hipGetLastError();
 }
 
 template <int MODE>
@@ -1121,9 +1377,14 @@ template <int MODE>
 hipError_t Launch<MODE>::ghost(const GhostArgs &a, int lay, hipStream_t s) {
     if (a.n_halo <= 0) return hipSuccess;
     const unsigned nb = blocks_for(a.n_halo, 256);
-    if (lay == LAY_P) k_ghost<MODE, LAY_P><<<nb, 256, 0, s>>>(a);
-    else if (lay == LAY_Q) k_ghost<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
-    else k_ghost<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
+    if constexpr (MODE == M_B16) {
+        if (lay != LAY_AB) return hipErrorNotSupported;
+        k_ghost<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
+    } else {
+        if (lay == LAY_P) k_ghost<MODE, LAY_P><<<nb, 256, 0, s>>>(a);
+        else if (lay == LAY_Q) k_ghost<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
+        else k_ghost<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
+    }
     return hipGetLastError();
 }
 
@@ -1137,10 +1398,15 @@ hipError_t Launch<MODE>::solid(const SolidArgs &a, hipStream_t s) {
 template <int MODE>
 hipError_t Launch<MODE>::source(const SourceArgs &a, int lay, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    const unsigned nb = blocks_for((long long)a.n * Q, 256);
-    if (lay == LAY_P) k_source<MODE, LAY_P><<<nb, 256, 0, s>>>(a);
-    else if (lay == LAY_Q) k_source<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
-    else k_source<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
+    const unsigned nb = blocks_for((long long)a.n * (MODE == M_B16 ? 1 : Q), 256);
+    if constexpr (MODE == M_B16) {
+        if (lay != LAY_AB) return hipErrorNotSupported;
+        k_source<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
+    } else {
+        if (lay == LAY_P) k_source<MODE, LAY_P><<<nb, 256, 0, s>>>(a);
+        else if (lay == LAY_Q) k_source<MODE, LAY_Q><<<nb, 256, 0, s>>>(a);
+        else k_source<MODE, LAY_AB><<<nb, 256, 0, s>>>(a);
+    }
     return hipGetLastError();
 }
 
@@ -1170,17 +1436,26 @@ hipError_t Launch<MODE>::prime_walls(void *pop, const uint8_t *mask, const GridD
                                      hipStream_t s) {
     const long long total = (long long)g.nx * g.ny * g.nz;
     if (lay != LAY_P) return hipErrorInvalidValue;
-    k_prime_walls<MODE><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, alpha);
-    return hipGetLastError();
+    if constexpr (MODE == M_B16) {
+        return hipErrorNotSupported;
+    } else {
+        k_prime_walls<MODE><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, alpha);
+        return hipGetLastError();
+    }
 }
 
 template <int MODE>
 hipError_t Launch<MODE>::fb_gather(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
                                    void *fb0, void *fb1, float scale, float inv_scale, hipStream_t s) {
     const long long n = 2LL * g.ny * g.nz;
-    if (lay == LAY_P) k_fb_gather<MODE, LAY_P><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
-    else if (lay == LAY_Q) k_fb_gather<MODE, LAY_Q><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
-    else k_fb_gather<MODE, LAY_AB><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+    if constexpr (MODE == M_B16) {
+        if (lay != LAY_AB) return hipErrorNotSupported;
+        k_fb_gather<MODE, LAY_AB><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+    } else {
+        if (lay == LAY_P) k_fb_gather<MODE, LAY_P><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+        else if (lay == LAY_Q) k_fb_gather<MODE, LAY_Q><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+        else k_fb_gather<MODE, LAY_AB><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
+    }
     return hipGetLastError();
 }
 
@@ -1209,15 +1484,19 @@ hipError_t Launch<MODE>::solid_multi(const SolidArgs *scenes, const int2 *blocks
 template <int MODE>
 hipError_t Launch<MODE>::source_multi(const SourceMulti &m, long long max_entries, hipStream_t s) {
     if (max_entries <= 0) return hipSuccess;
-    k_source_multi<MODE, LAY_AB><<<blocks_for(max_entries * Q, 256), 256, 0, s>>>(m);
+    k_source_multi<MODE, LAY_AB><<<blocks_for(max_entries * (MODE == M_B16 ? 1 : Q), 256), 256, 0, s>>>(m);
     return hipGetLastError();
 }
 
 template <int MODE>
 hipError_t Launch<MODE>::adjoint(const AdjArgs &a, hipStream_t s) {
-    const long long total = (long long)a.g.nx * a.g.ny * a.g.nz;
-    k_adjoint<MODE><<<blocks_for(total, 256), 256, 0, s>>>(a);
-    return hipGetLastError();
+    if constexpr (MODE == M_B16 || MODE == M_F16) {
+        return hipErrorNotSupported;
+    } else {
+        const long long total = (long long)a.g.nx * a.g.ny * a.g.nz;
+        k_adjoint<MODE><<<blocks_for(total, 256), 256, 0, s>>>(a);
+        return hipGetLastError();
+    }
 }
 
 template <int MODE>
@@ -1240,9 +1519,14 @@ template <int MODE>
 hipError_t Launch<MODE>::gather_f(const void *pop, const uint8_t *mask, const GridDesc &g, int lay,
                                   float inv_scale, double *out, hipStream_t s) {
     const long long total = (long long)g.nx * g.ny * g.nz;
-    if (lay == LAY_P) k_gather_f<MODE, LAY_P><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
-    else if (lay == LAY_Q) k_gather_f<MODE, LAY_Q><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
-    else k_gather_f<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
+    if constexpr (MODE == M_B16) {
+        if (lay != LAY_AB) return hipErrorNotSupported;
+        k_gather_f<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
+    } else {
+        if (lay == LAY_P) k_gather_f<MODE, LAY_P><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
+        else if (lay == LAY_Q) k_gather_f<MODE, LAY_Q><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
+        else k_gather_f<MODE, LAY_AB><<<blocks_for(total, 256), 256, 0, s>>>(pop, mask, g, inv_scale, out);
+    }
     return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
 // Host-side launch interface of the per-storage-mode kernel translation units
-// (kernels_f64.hip, kernels_f32.hip, kernels_f16.hip).  The engine sees only these
+// (kernels_f64.hip, kernels_f32.hip, kernels_f16.hip, kernels_b16.hip).  The engine sees only these
 // declarations; each mode's TU instantiates Launch<MODE>.
 #pragma once
 

@@ -19,8 +19,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("NLOS_ENGINE_LIB") or os.path.join(HERE, "libnlos_engine.so")
 
-F64, F32, F16S = 0, 1, 2
-PRECISIONS = {"f64": F64, "f32": F32, "f16s": F16S}
+F64, F32, F16S, B16 = 0, 1, 2, 3
+PRECISIONS = {"f64": F64, "f32": F32, "f16s": F16S, "b16": B16}
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = 1, 2, 4, 8, 16, 32
 FACES_REFERENCE = FACE_XM | FACE_XP | FACE_YM | FACE_YP | FACE_ZP
 FACES_ALL = FACES_REFERENCE | FACE_ZM
@@ -160,7 +160,7 @@ def _check(rc: int) -> None:
 
 def default_config(**kw) -> Config:
     """05's member defaults (05:243-268, 455-458) with keyword overrides.  A keyword
-    `precision` may be "f64", "f32" or "f16s"."""
+    `precision` may be "f64", "f32", "f16s" or "b16"."""
     cfg = Config()
     _check(load_library().nlos_config_default(C.byref(cfg)))
     for k, v in kw.items():

@@ -51,8 +51,12 @@ typedef enum nlos_status {
 /* Population storage.  NLOS_F64 keeps full populations f in FP64 and follows the
  * reference's arithmetic operation for operation.  NLOS_F32 and NLOS_F16S store the
  * deviation g = f - w_q (rest density 1) in FP32 or in FP16 scaled by 2^f16_scale_log2, and
- * do all collision arithmetic in FP32. */
-typedef enum nlos_precision { NLOS_F64 = 0, NLOS_F32 = 1, NLOS_F16S = 2 } nlos_precision;
+ * do all collision arithmetic in FP32.  NLOS_B16 stores the same deviations in 16 bits per
+ * population as block floating point: a 16-bit mantissa of g_q / w_q per population and one
+ * exponent byte per cell (39 bytes per cell instead of FP16's 38), which keeps 15
+ * significant bits relative to each cell's largest value; FP32 arithmetic; two-array
+ * stream pattern only.  No counterpart in the reference (it stores FP64 only). */
+typedef enum nlos_precision { NLOS_F64 = 0, NLOS_F32 = 1, NLOS_F16S = 2, NLOS_B16 = 3 } nlos_precision;
 
 /* Domain faces, used for sponge and NSCBC face sets. */
 enum {

@@ -53,7 +53,7 @@ def _single(d, steps):
     return res
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32", "f16s"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f16s", "b16"])
 def test_batch_equals_scenes_stepped_alone(gpu, precision):
     print(f"[batch vs single] precision={precision}", flush=True)
     defs = _defs(precision)
