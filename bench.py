@@ -183,9 +183,9 @@ def bench_scatter(args) -> None:
 
 
 def bench_city2048(args) -> None:
-    """Precision sweep at dx = 0.5 m: FP16S and FP32 MLUPS on the 2048x2048x512 city (FP64
-    does not fit), then per-probe error of FP16S/FP32 against FP64 on a 512x512x256 sub-box
-    of the same construction (all three precisions on identical geometry and source)."""
+    """Precision sweep at dx = 0.5 m: FP16S, B16 and FP32 MLUPS on the 2048x2048x512 city
+    (FP64 does not fit), then per-probe error of each against FP64 on a 512x512x256 sub-box
+    of the same construction (all precisions on identical geometry and source)."""
     steps = args.steps if args.steps != 200 else 40
     warmup = min(args.warmup, 5)
     wl = workloads.CityWorkload(nx=2048, ny=2048, nz=512, dx=0.5, sponge_cells=64, n_probes=128)
@@ -193,7 +193,7 @@ def bench_city2048(args) -> None:
     stl = wl.stl()
     log(f"city2048: {len(wl.boxes)} boxes ({time.perf_counter() - t:.1f} s)")
     res = {}
-    for p in ("f16s", "f32"):
+    for p in ("f16s", "b16", "f32"):
         log(f"running {p}: warmup {warmup}, timed {steps}")
         r = run_precision(wl, stl, p, warmup, steps, pattern=args.pattern)
         r.pop("mask")
@@ -217,14 +217,15 @@ def bench_city2048(args) -> None:
         sub = workloads.CityWorkload(nx=512, ny=512, nz=256, dx=0.5, sponge_cells=64, n_probes=128)
         sstl = sub.stl()
         traces = {}
-        for p in ("f64", "f32", "f16s"):
+        for p in ("f64", "f32", "f16s", "b16"):
             log(f"sub-box error run {p}: {n_err} steps")
             traces[p] = run_precision(sub, sstl, p, 0, n_err, timing=False)["probes"]
         ref = traces["f64"]
         err = {"steps": n_err, "sub_box": "512x512x256 at dx=0.5 m", "signal_max": float(np.abs(ref).max())}
-        for p in ("f32", "f16s"):
+        for p in ("f32", "f16s", "b16"):
             per = [rel_l2(a, b) for a, b in zip(traces[p], ref) if np.abs(b).max() > 0]
-            err[p] = {"rel_l2": rel_l2(traces[p], ref), "per_probe_rel_l2": [round(x, 6) for x in per]}
+            err[p] = {"rel_l2": rel_l2(traces[p], ref), "median_probe_rel_l2": float(np.median(per)),
+                      "max_probe_rel_l2": max(per), "per_probe_rel_l2": [round(x, 6) for x in per]}
         out["probe_error_vs_f64"] = err
     print(json.dumps(out), flush=True)
 
