@@ -423,6 +423,51 @@ def bench_adjoint(args) -> None:
     print(json.dumps(out), flush=True)
 
 
+def bench_frames(args) -> None:
+    """VTK frames during a run (05:1095-1103): the same city run three ways, steps only,
+    frames written synchronously, frames dumped off the step stream (copy on a second
+    stream, file written by a host thread while the next steps run).  Binary legacy VTK
+    frames into a scratch directory that is removed afterwards."""
+    import shutil
+    import tempfile
+    nx, ny, nz = args.nx, args.ny, args.nz
+    wl = workloads.CityWorkload(nx=nx, ny=ny, nz=nz)
+    stl = wl.stl()
+    steps = args.steps if args.steps != 200 else 600
+    frames = 3
+    root = tempfile.mkdtemp(prefix="nlos_frames_")
+    out = {"metric": "wall time of a run with VTK frames (binary), synchronous vs off-stream dump",
+           "unit": "s", "higher_is_better": False, "n_gpus": 1, "steps": steps, "frames": frames,
+           "data": "synthetic",
+           "config": {"workload": f"moving-source city {nx}x{ny}x{nz}, FP16S two-array, "
+                                  f"{frames} binary VTK frames of {nx * ny * nz * 4 / 1e9:.2f} GB"}}
+    try:
+        for mode in ("steps_only", "sync", "async"):
+            e = nlos.Engine(wl.config("f16s"))
+            e.voxelize_stl(stl)
+            e.set_source(wl.source())
+            e.set_probes(wl.probes(e.ground_heights()))
+            e.step(5)
+            e.synchronize()
+            d = os.path.join(root, mode)
+            t = time.perf_counter()
+            total_t = (steps - 1) * e.dt
+            e.run(total_t, frames, vtk_dir=None if mode == "steps_only" else d,
+                  async_frames=(mode == "async"), binary_frames=True)
+            e.synchronize()
+            out[f"{mode}_s"] = round(time.perf_counter() - t, 3)
+            log(f"  {mode}: {out[f'{mode}_s']} s")
+            e.close()
+            shutil.rmtree(d, ignore_errors=True)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    out["value"] = out["async_s"]
+    # the run starts after 5 warm-up steps, so its frames fall at steps 199, 398, 597
+    out["frame_cost_sync_s"] = round((out["sync_s"] - out["steps_only_s"]) / frames, 3)
+    out["frame_cost_async_s"] = round((out["async_s"] - out["steps_only_s"]) / frames, 3)
+    print(json.dumps(out), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -438,12 +483,15 @@ def main() -> None:
     ap.add_argument("--pattern", default="auto", choices=["auto", "inplace", "two_array"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 2 GiB device copy kernel (PMC byte calibration)")
-    ap.add_argument("--workload", default="city", choices=["city", "scatter", "city2048", "batch", "adjoint"],
+    ap.add_argument("--workload", default="city",
+                    choices=["city", "scatter", "city2048", "batch", "adjoint", "frames"],
                     help="city: the headline moving-source city step (default); scatter: the "
                          "128^3 rigid-cube case, FP64 against the oracle; city2048: the dx=0.5 m "
                          "precision sweep on 2048x2048x512 with its 512x512x256 error sub-box; "
                          "batch: 256 small FP32 scenes advanced together; adjoint: source-"
-                         "localisation gradient in a 256x256x128 canyon against finite differences")
+                         "localisation gradient in a 256x256x128 canyon against finite differences; "
+                         "frames: a city run writing binary VTK frames synchronously and off the "
+                         "step stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -459,6 +507,8 @@ def main() -> None:
         return bench_batch(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "frames":
+        return bench_frames(args)
 
     wl = workloads.CityWorkload(nx=args.nx, ny=args.ny, nz=args.nz)
     t = time.perf_counter()

@@ -184,6 +184,15 @@ struct nlos_engine {
     std::vector<long long> sched_cells;
     std::vector<double> sched_adds;
 
+    // field dump off the step stream (nlos_dump_field_async)
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t dump_ready = nullptr, dump_copied = nullptr;
+    float *dump_host = nullptr;       // pinned, n_interior floats
+    std::thread dump_thread;
+    std::string dump_path, dump_msg;
+    int dump_format = 0, dump_rc = 0;
+    bool dump_pending = false;        // a capture must wait on the device for dump_copied
+
     long long n_interior() const { return (long long)g.nx * g.ny * g.nz; }
     void *curp() const { return cur == 0 ? pop.p : pop2.p; }
     void *nxtp() const { return cur == 0 ? pop2.p : pop.p; }
@@ -611,6 +620,8 @@ int launch_halos(nlos_engine *e) {
 
 int launch_sweep(nlos_engine *e, bool capture) {
     if (capture) {
+        // the field buffer may still be on its way to the host for a dump
+        if (e->dump_pending) HIP_TRY(hipStreamWaitEvent(e->stream, e->dump_copied, 0));
         if (!e->field_alloc) {
             HIP_TRY(e->field.reserve((size_t)e->n_interior() * 4));
             e->field_alloc = true;
@@ -821,7 +832,13 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
 NLOS_API int nlos_destroy(nlos_engine *e) {
     if (!e) return NLOS_OK;
     if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; destroy the batch");
+    if (e->dump_thread.joinable()) e->dump_thread.join();
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->copy_stream) (void)hipStreamSynchronize(e->copy_stream);
+    if (e->dump_host) (void)hipHostFree(e->dump_host);
+    if (e->dump_ready) (void)hipEventDestroy(e->dump_ready);
+    if (e->dump_copied) (void)hipEventDestroy(e->dump_copied);
+    if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
     DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                       &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
                       &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
@@ -1123,27 +1140,125 @@ NLOS_API int nlos_write_probe_files(nlos_engine *e, const char *dir, const char 
     return NLOS_OK;
 }
 
-NLOS_API int nlos_write_vtk(nlos_engine *e, const char *path) {
-    if (int rc = ready(e)) return rc;
-    if (!path) return fail(NLOS_ERR_ARG, "null path");
+namespace {
+
+// The VTK file of a captured field v (interior, x fastest).  ASCII follows 05:1045-1053:
+// default stream formatting (%g) for the header, "%.6e" per value.  BINARY is the legacy
+// VTK binary encoding of the same dataset (big-endian float32 after the header).
+int write_vtk_host(const nlos_engine *e, const char *path, const float *v, int format) {
     const long long n = e->n_interior();
-    std::vector<float> v((size_t)n);
-    if (int rc = nlos_read_field(e, v.data())) return rc;
-    FILE *f = std::fopen(path, "w");
+    FILE *f = std::fopen(path, format == NLOS_VTK_BINARY ? "wb" : "w");
     if (!f) return fail(NLOS_ERR_IO, "cannot write %s", path);
+    std::vector<char> iobuf((size_t)1 << 22);
+    std::setvbuf(f, iobuf.data(), _IOFBF, iobuf.size());
     double ox = e->cfg.x_min, oy = e->cfg.y_min, oz = e->cfg.z_min;
     if (e->cfg.output_world_coords) { ox += e->cfg.world_offset[0]; oy += e->cfg.world_offset[1]; oz += e->cfg.world_offset[2]; }
-    // 05:1045-1053: default stream formatting is %g; values in scientific, precision 6.
-    std::fprintf(f, "# vtk DataFile Version 3.0\nLBM Urban Acoustics V21\nASCII\n");
+    std::fprintf(f, "# vtk DataFile Version 3.0\nLBM Urban Acoustics V21\n%s\n",
+                 format == NLOS_VTK_BINARY ? "BINARY" : "ASCII");
     std::fprintf(f, "DATASET STRUCTURED_POINTS\n");
     std::fprintf(f, "DIMENSIONS %d %d %d\n", e->g.nx, e->g.ny, e->g.nz);
     std::fprintf(f, "ORIGIN %g %g %g\n", ox, oy, oz);
     std::fprintf(f, "SPACING %g %g %g\n", e->cfg.dx, e->cfg.dx, e->cfg.dx);
     std::fprintf(f, "POINT_DATA %zu\n", (size_t)n);
     std::fprintf(f, "SCALARS PressureAmp float 1\nLOOKUP_TABLE default\n");
-    std::vector<char> line(64);
-    for (long long q = 0; q < n; ++q) std::fprintf(f, "%.6e\n", (double)v[(size_t)q]);
-    std::fclose(f);
+    bool ok = true;
+    if (format == NLOS_VTK_BINARY) {
+        std::vector<uint32_t> be((size_t)1 << 16);
+        for (long long q0 = 0; q0 < n && ok; q0 += (long long)be.size()) {
+            const size_t m = (size_t)std::min<long long>((long long)be.size(), n - q0);
+            for (size_t r = 0; r < m; ++r) {
+                uint32_t u;
+                std::memcpy(&u, v + q0 + (long long)r, 4);
+                be[r] = __builtin_bswap32(u);
+            }
+            ok = std::fwrite(be.data(), 4, m, f) == m;
+        }
+        ok = ok && std::fputc('\n', f) != EOF;
+    } else {
+        for (long long q = 0; q < n; ++q) std::fprintf(f, "%.6e\n", (double)v[(size_t)q]);
+    }
+    ok = !std::ferror(f) && ok;
+    ok = std::fclose(f) == 0 && ok;
+    return ok ? NLOS_OK : fail(NLOS_ERR_IO, "write failed: %s", path);
+}
+
+int write_vtk_sync(nlos_engine *e, const char *path, int format) {
+    if (int rc = ready(e)) return rc;
+    if (!path) return fail(NLOS_ERR_ARG, "null path");
+    std::vector<float> v((size_t)e->n_interior());
+    if (int rc = nlos_read_field(e, v.data())) return rc;
+    return write_vtk_host(e, path, v.data(), format);
+}
+
+}  // namespace
+
+NLOS_API int nlos_write_vtk(nlos_engine *e, const char *path) { return write_vtk_sync(e, path, NLOS_VTK_ASCII); }
+
+NLOS_API int nlos_write_vtk_binary(nlos_engine *e, const char *path) {
+    return write_vtk_sync(e, path, NLOS_VTK_BINARY);
+}
+
+NLOS_API int nlos_dump_wait(nlos_engine *e) {
+    if (int rc = ready(e)) return rc;
+    if (e->dump_thread.joinable()) e->dump_thread.join();
+    e->dump_pending = false;
+    if (e->dump_rc != NLOS_OK) {
+        const int rc = e->dump_rc;
+        e->dump_rc = NLOS_OK;
+        return fail(rc, "field dump: %s", e->dump_msg.c_str());
+    }
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_dump_field_async(nlos_engine *e, const char *path, int32_t format) {
+    if (int rc = ready(e)) return rc;
+    if (!path) return fail(NLOS_ERR_ARG, "null path");
+    if (format != NLOS_VTK_ASCII && format != NLOS_VTK_BINARY) return fail(NLOS_ERR_ARG, "bad VTK format %d", format);
+    if (!e->field_valid) return fail(NLOS_ERR_STATE, "no field captured (step with NLOS_STEP_CAPTURE_FIELD)");
+    if (int rc = nlos_dump_wait(e)) return rc;
+    const size_t bytes = (size_t)e->n_interior() * 4;
+    if (!e->copy_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&e->dump_ready, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&e->dump_copied, hipEventDisableTiming));
+    }
+    if (!e->dump_host) HIP_TRY(hipHostMalloc((void **)&e->dump_host, bytes, hipHostMallocDefault));
+    HIP_TRY(hipEventRecord(e->dump_ready, e->stream));
+    HIP_TRY(hipStreamWaitEvent(e->copy_stream, e->dump_ready, 0));
+    HIP_TRY(hipMemcpyAsync(e->dump_host, e->field.p, bytes, hipMemcpyDeviceToHost, e->copy_stream));
+    HIP_TRY(hipEventRecord(e->dump_copied, e->copy_stream));
+    e->dump_pending = true;
+    e->dump_path = path;
+    e->dump_format = format;
+    e->dump_thread = std::thread([e]() {
+        const hipError_t he = hipEventSynchronize(e->dump_copied);
+        if (he != hipSuccess) {
+            e->dump_rc = NLOS_ERR_HIP;
+            e->dump_msg = hipGetErrorString(he);
+            return;
+        }
+        e->dump_rc = write_vtk_host(e, e->dump_path.c_str(), e->dump_host, e->dump_format);
+        if (e->dump_rc != NLOS_OK) e->dump_msg = g_err;   // this thread's message
+    });
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_write_probe_dat(nlos_engine *e, const char *dir, const char *const *names, double scale) {
+    if (int rc = ready(e)) return rc;
+    if (!names) return fail(NLOS_ERR_ARG, "null names");
+    const long long ns = e->step;
+    std::vector<double> buf((size_t)e->probes.size() * std::max<long long>(ns, 1));
+    int64_t got = 0;
+    if (int rc = nlos_read_probes(e, buf.data(), ns, &got)) return rc;
+    for (size_t p = 0; p < e->probes.size(); ++p) {
+        std::string path = std::string(dir && *dir ? dir : ".") + "/probe_" + names[p] + ".dat";
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f) return fail(NLOS_ERR_IO, "cannot write %s", path.c_str());
+        std::fprintf(f, "# Time Pressure\n");                                        // 01:487
+        for (long long s = 0; s < got; ++s)                                          // 01:488-491
+            std::fprintf(f, "%g %g\n", (double)(s + 1) * e->dt, buf[(size_t)p * ns + s] * scale);
+        if (std::fclose(f) != 0) return fail(NLOS_ERR_IO, "write failed: %s", path.c_str());
+    }
     return NLOS_OK;
 }
 

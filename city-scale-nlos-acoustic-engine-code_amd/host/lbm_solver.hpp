@@ -71,6 +71,8 @@ public:
     int waveform = NLOS_WAVE_GAUSS_PULSE;
     int injection = NLOS_INJECT_GAUSS5;
     bool write_frames = true;
+    bool async_frames = true;         // frames written off the step stream (same files)
+    bool binary_frames = false;       // legacy VTK BINARY instead of the reference's ASCII
 
     LBMSolver() = default;
     LBMSolver(const LBMSolver &) = delete;
@@ -138,7 +140,9 @@ public:
         std::filesystem::create_directories("output_mpi");
         char fn[64];
         std::snprintf(fn, sizeof fn, "output_mpi/flight_%05d.vtk", frame_id);
-        check(nlos_write_vtk(eng_, fn));
+        const int fmt = binary_frames ? NLOS_VTK_BINARY : NLOS_VTK_ASCII;
+        if (async_frames) check(nlos_dump_field_async(eng_, fn, fmt));
+        else check(binary_frames ? nlos_write_vtk_binary(eng_, fn) : nlos_write_vtk(eng_, fn));
     }
 
     // LBMSolver::run (05:1064-1121).
@@ -158,6 +162,7 @@ public:
             s = next + 1;
             if (frame) write_vtk(frame_id++);
         }
+        check(nlos_dump_wait(eng_));
         collect_probes();
         for (auto &p : probes) {
             FILE *f = std::fopen(("probe_" + p.name + ".txt").c_str(), "w");

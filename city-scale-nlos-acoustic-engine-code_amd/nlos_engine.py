@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("NLOS_ENGINE_LIB") or os.path.join(HERE, "libnlos_engine.so")
 
 F64, F32, F16S, B16 = 0, 1, 2, 3
+VTK_ASCII, VTK_BINARY = 0, 1
 PRECISIONS = {"f64": F64, "f32": F32, "f16s": F16S, "b16": B16}
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = 1, 2, 4, 8, 16, 32
 FACES_REFERENCE = FACE_XM | FACE_XP | FACE_YM | FACE_YP | FACE_ZP
@@ -107,6 +108,10 @@ SIGNATURES = {
     "nlos_read_field": (C.c_int, [_E, _P]),
     "nlos_write_probe_files": (C.c_int, [_E, C.c_char_p, C.POINTER(C.c_char_p)]),
     "nlos_write_vtk": (C.c_int, [_E, C.c_char_p]),
+    "nlos_write_vtk_binary": (C.c_int, [_E, C.c_char_p]),
+    "nlos_dump_field_async": (C.c_int, [_E, C.c_char_p, C.c_int32]),
+    "nlos_dump_wait": (C.c_int, [_E]),
+    "nlos_write_probe_dat": (C.c_int, [_E, C.c_char_p, C.POINTER(C.c_char_p), C.c_double]),
     "nlos_checkpoint": (C.c_int, [_E, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "nlos_restore": (C.c_int, [_E, _P, C.c_size_t]),
     "nlos_timing_enable": (C.c_int, [_E, C.c_int32]),
@@ -300,9 +305,12 @@ class Engine:
         _check(self.lib.nlos_synchronize(self.h))
 
     def run(self, T_total: float, Frames: int, vtk_dir: Optional[str] = None,
-            probe_dir: Optional[str] = None) -> int:
+            probe_dir: Optional[str] = None, async_frames: bool = True,
+            binary_frames: bool = False) -> int:
         """LBMSolver::run (05:1064-1121): total_steps = round(T/dt), steps s = 0..total,
-        a VTK frame every out_interval steps, probe files at the end."""
+        a VTK frame every out_interval steps, probe files at the end.  With async_frames a
+        frame is copied and written off the step stream while the next steps run (the files
+        are the same); binary_frames writes legacy VTK BINARY instead of the ASCII text."""
         dt = self.dt
         total = int(round(T_total / dt))
         out_interval = max(1, total // Frames)
@@ -316,8 +324,14 @@ class Engine:
             self.step(f - s + 1, capture_field=True)
             s = f + 1
             os.makedirs(vtk_dir, exist_ok=True)
-            self.write_vtk(os.path.join(vtk_dir, f"flight_{frame:05d}.vtk"))
+            path = os.path.join(vtk_dir, f"flight_{frame:05d}.vtk")
+            if async_frames:
+                self.dump_field_async(path, binary=binary_frames)
+            else:
+                self.write_vtk(path, binary=binary_frames)
             frame += 1
+        if async_frames:
+            self.dump_wait()
         if probe_dir is not None and self.probe_names:
             self.write_probe_files(probe_dir)
         return total
@@ -341,8 +355,21 @@ class Engine:
         names = (C.c_char_p * len(self.probe_names))(*[n.encode() for n in self.probe_names])
         _check(self.lib.nlos_write_probe_files(self.h, directory.encode(), names))
 
-    def write_vtk(self, path: str) -> None:
-        _check(self.lib.nlos_write_vtk(self.h, path.encode()))
+    def write_vtk(self, path: str, binary: bool = False) -> None:
+        fn = self.lib.nlos_write_vtk_binary if binary else self.lib.nlos_write_vtk
+        _check(fn(self.h, path.encode()))
+
+    def dump_field_async(self, path: str, binary: bool = False) -> None:
+        """Write the captured field to path off the step stream (see nlos_dump_field_async)."""
+        _check(self.lib.nlos_dump_field_async(self.h, path.encode(), VTK_BINARY if binary else VTK_ASCII))
+
+    def dump_wait(self) -> None:
+        _check(self.lib.nlos_dump_wait(self.h))
+
+    def write_probe_dat(self, directory: str, scale: float = 100.0) -> None:
+        """probe_<name>.dat in the family-A format (01_acoustic_scattering.cpp:481-495)."""
+        names = (C.c_char_p * len(self.probe_names))(*[n.encode() for n in self.probe_names])
+        _check(self.lib.nlos_write_probe_dat(self.h, directory.encode(), names, scale))
 
     def checkpoint(self) -> bytes:
         need = C.c_size_t()

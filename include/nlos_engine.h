@@ -202,6 +202,29 @@ int nlos_write_probe_files(nlos_engine *e, const char *dir, const char *const *n
 /* Legacy ASCII VTK of the captured field, byte-compatible with 05:1045-1053. */
 int nlos_write_vtk(nlos_engine *e, const char *path);
 
+/* VTK encodings of the captured field.  NLOS_VTK_ASCII is the reference's text
+ * (05:1045-1053); NLOS_VTK_BINARY is the legacy VTK BINARY form of the same dataset: the
+ * header with "BINARY" in place of "ASCII", then the nx*ny*nz float32 values big-endian,
+ * x fastest.  No counterpart in the reference (it writes ASCII only). */
+#define NLOS_VTK_ASCII 0
+#define NLOS_VTK_BINARY 1
+int nlos_write_vtk_binary(nlos_engine *e, const char *path);
+
+/* Field dump off the step stream: copies the captured field to pinned host memory on a
+ * second stream and writes the file (NLOS_VTK_ASCII or NLOS_VTK_BINARY) on a host thread,
+ * returning at once.  Steps keep running meanwhile; a later step that captures the field
+ * waits on the device for the copy to finish.  One dump is in flight per engine (a second
+ * call first waits for the first).  nlos_dump_wait waits for it and returns its status; the
+ * reference writes frames synchronously inside run (05:1095-1103). */
+int nlos_dump_field_async(nlos_engine *e, const char *path, int32_t format);
+int nlos_dump_wait(nlos_engine *e);
+
+/* Probe histories in the family-A text format of 01_acoustic_scattering.cpp:481-495:
+ * dir/probe_<name>.dat, a "# Time Pressure" line, then one "%g %g" line per recorded step,
+ * time = s*dt after s steps (s = 1, 2, ...) and pressure = scale*(rho-1) (01:466, whose
+ * output_scale_factor is 100).  The probe cells are the engine's (05:415-437). */
+int nlos_write_probe_dat(nlos_engine *e, const char *dir, const char *const *names, double scale);
+
 /* ---- checkpoint / restore ------------------------------------------------------------ */
 /* Serialises populations, tracked boundary-solid state, step index and probe histories.
  * With buf == NULL only *needed is set. */
