@@ -72,6 +72,26 @@ struct HostGeometry {
     double cell_w = 1.0, cell_h = 1.0;
     std::vector<int> bin_start, bin_tris;
 
+    // One 50-byte record (normal, 3 vertices as float32, attribute), transformed as in
+    // 05:87-90, with the bounding boxes of 05:92-104.
+    void add_record(const unsigned char *r, size_t t, double scale, const double sh[3]) {
+        float rec[12];
+        std::memcpy(rec, r, 48);
+        double v[9];
+        for (int p = 0; p < 3; ++p)
+            for (int a = 0; a < 3; ++a) v[3 * p + a] = rec[3 + 3 * p + a] * scale + sh[a];
+        Triangle13 &T = tris[t];
+        for (int a = 0; a < 9; ++a) T.v[a] = v[a];
+        double mn[3], mx[3];
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min({v[a], v[3 + a], v[6 + a]});
+            mx[a] = std::max({v[a], v[3 + a], v[6 + a]});
+            gmin[a] = std::min(gmin[a], mn[a]);
+            gmax[a] = std::max(gmax[a], mx[a]);
+        }
+        T.v[9] = mn[0]; T.v[10] = mn[1]; T.v[11] = mx[0]; T.v[12] = mx[1];
+    }
+
     bool parse(const unsigned char *buf, size_t len, double scale, double sx, double sy, double sz) {
         tris.clear();
         if (len < 84) return false;
@@ -80,26 +100,81 @@ struct HostGeometry {
         if (len < 84 + (size_t)n * 50) return false;
         tris.resize(n);
         for (int a = 0; a < 3; ++a) { gmin[a] = 1e30; gmax[a] = -1e30; }
-        for (uint32_t t = 0; t < n; ++t) {
-            float rec[12];
-            std::memcpy(rec, buf + 84 + (size_t)t * 50, 48);
-            double v[9];
-            const double sh[3] = {sx, sy, sz};
-            for (int p = 0; p < 3; ++p)
-                for (int a = 0; a < 3; ++a) v[3 * p + a] = rec[3 + 3 * p + a] * scale + sh[a];
-            Triangle13 &T = tris[t];
-            for (int a = 0; a < 9; ++a) T.v[a] = v[a];
-            double mn[3], mx[3];
-            for (int a = 0; a < 3; ++a) {
-                mn[a] = std::min({v[a], v[3 + a], v[6 + a]});
-                mx[a] = std::max({v[a], v[3 + a], v[6 + a]});
-                gmin[a] = std::min(gmin[a], mn[a]);
-                gmax[a] = std::max(gmax[a], mx[a]);
-            }
-            T.v[9] = mn[0]; T.v[10] = mn[1]; T.v[11] = mx[0]; T.v[12] = mx[1];
-        }
+        const double sh[3] = {sx, sy, sz};
+        for (uint32_t t = 0; t < n; ++t) add_record(buf + 84 + (size_t)t * 50, t, scale, sh);
         build_bins();
         return true;
+    }
+
+    // The same from a file, read in chunks of records (the whole file is never held;
+    // GeometryEngine::load reads record by record too, 05:70-111).  False when the file
+    // cannot be opened or holds fewer records than its count.
+    bool parse_file(const char *path, double scale, double sx, double sy, double sz) {
+        tris.clear();
+        FILE *f = std::fopen(path, "rb");
+        if (!f) return false;
+        unsigned char head[84];
+        bool ok = std::fread(head, 1, 84, f) == 84;
+        uint32_t n = 0;
+        if (ok) std::memcpy(&n, head + 80, 4);
+        if (ok) {
+            tris.resize(n);
+            for (int a = 0; a < 3; ++a) { gmin[a] = 1e30; gmax[a] = -1e30; }
+            const double sh[3] = {sx, sy, sz};
+            constexpr size_t CHUNK = 1 << 16;
+            std::vector<unsigned char> buf(CHUNK * 50);
+            for (size_t t0 = 0; t0 < n && ok; t0 += CHUNK) {
+                const size_t m = std::min<size_t>(CHUNK, n - t0);
+                ok = std::fread(buf.data(), 50, m, f) == m;
+                for (size_t r = 0; ok && r < m; ++r) add_record(buf.data() + r * 50, t0 + r, scale, sh);
+            }
+        }
+        std::fclose(f);
+        if (!ok) { tris.clear(); return false; }
+        build_bins();
+        return true;
+    }
+
+    // CSR bins of the triangles over the plane (b, c) = ((A+1)%3, (A+2)%3) normal to axis A,
+    // rows x cols over the mesh's extent there (for the crossing voxeliser).
+    struct AxisBins {
+        double bmin = 0, bmax = 0, cmin = 0, cmax = 0, cell_b = 1, cell_c = 1;
+        std::vector<int> start, list;
+    };
+    AxisBins axis_bins(int A, int nr = 200, int nc = 200) const {
+        AxisBins o;
+        const int B = (A + 1) % 3, C = (A + 2) % 3;
+        o.bmin = gmin[B]; o.bmax = gmax[B]; o.cmin = gmin[C]; o.cmax = gmax[C];
+        o.cell_b = (o.bmax - o.bmin) / nr;
+        o.cell_c = (o.cmax - o.cmin) / nc;
+        if (o.cell_b < 1e-9) o.cell_b = 1.0;
+        if (o.cell_c < 1e-9) o.cell_c = 1.0;
+        o.start.assign((size_t)nr * nc + 1, 0);
+        auto range = [&](const Triangle13 &T, int &r1, int &r2, int &c1, int &c2) {
+            const double b0 = std::min({T.v[B], T.v[3 + B], T.v[6 + B]}), b1 = std::max({T.v[B], T.v[3 + B], T.v[6 + B]});
+            const double c0 = std::min({T.v[C], T.v[3 + C], T.v[6 + C]}), c1v = std::max({T.v[C], T.v[3 + C], T.v[6 + C]});
+            r1 = std::max(0, std::min(nr - 1, (int)((b0 - o.bmin) / o.cell_b)));
+            r2 = std::max(0, std::min(nr - 1, (int)((b1 - o.bmin) / o.cell_b)));
+            c1 = std::max(0, std::min(nc - 1, (int)((c0 - o.cmin) / o.cell_c)));
+            c2 = std::max(0, std::min(nc - 1, (int)((c1v - o.cmin) / o.cell_c)));
+        };
+        std::vector<int> count((size_t)nr * nc, 0);
+        for (const Triangle13 &T : tris) {
+            int r1, r2, c1, c2;
+            range(T, r1, r2, c1, c2);
+            for (int r = r1; r <= r2; ++r)
+                for (int c = c1; c <= c2; ++c) ++count[(size_t)r * nc + c];
+        }
+        for (size_t b = 0; b < count.size(); ++b) o.start[b + 1] = o.start[b] + count[b];
+        o.list.resize(o.start.back());
+        std::vector<int> fill(o.start.begin(), o.start.end() - 1);
+        for (size_t t = 0; t < tris.size(); ++t) {
+            int r1, r2, c1, c2;
+            range(tris[t], r1, r2, c1, c2);
+            for (int r = r1; r <= r2; ++r)
+                for (int c = c1; c <= c2; ++c) o.list[fill[(size_t)r * nc + c]++] = (int)t;
+        }
+        return o;
     }
 
     void build_bins() {
@@ -874,13 +949,16 @@ NLOS_API int nlos_get_info(const nlos_engine *e, nlos_info *info) {
 }
 
 // ---- geometry --------------------------------------------------------------------------
-NLOS_API int nlos_voxelize_stl(nlos_engine *e, const void *stl, size_t len, double scale, double sx,
-                               double sy, double sz) {
-    if (int rc = ready(e)) return rc;
-    if (e->step != 0) return fail(NLOS_ERR_STATE, "geometry can only change before the first step");
-    HostGeometry geo;
-    const bool ok = stl && geo.parse((const unsigned char *)stl, len, scale, sx, sy, sz);
-    if (!ok) geo = HostGeometry();
+namespace {
+
+// Mask (and column heights) from a parsed mesh; ok = false means the mesh could not be read
+// and empty geometry is used, reported as NLOS_ERR_IO like 05:407-410.
+int voxelize_geometry(nlos_engine *e, HostGeometry &geo, bool ok) {
+    const int mode = e->cfg.voxelizer;
+    const int kind = mode & 3;
+    const bool three = (mode & NLOS_VOXEL_3AXIS) != 0;
+    if (kind > 2 || (mode & ~7) != 0 || (kind == NLOS_VOXEL_HEIGHT && three))
+        return fail(NLOS_ERR_ARG, "bad voxelizer %d", mode);
     VoxelArgs v{};
     v.n_tri = (int)geo.tris.size();
     v.grid_rows = geo.rows; v.grid_cols = geo.cols;
@@ -901,7 +979,53 @@ NLOS_API int nlos_voxelize_stl(nlos_engine *e, const void *stl, size_t len, doub
         v.bin_tris = e->vox_bins.as<int>();
     }
     HIP_TRY(launch_column_heights(v, e->g, e->gh.as<double>(), e->stream));
-    HIP_TRY(launch_mask_from_heights(v, e->g, e->gh.as<double>(), e->mask.as<uint8_t>(), e->stream));
+    if (kind == NLOS_VOXEL_HEIGHT) {
+        HIP_TRY(launch_mask_from_heights(v, e->g, e->gh.as<double>(), e->mask.as<uint8_t>(), e->stream));
+    } else {
+        const size_t n = (size_t)e->n_interior();
+        DevBuf votes, flag, bstart, blist;
+        HIP_TRY(votes.reserve(n));
+        HIP_TRY(flag.reserve(4));
+        HIP_TRY(launch_fill_u8(votes.as<uint8_t>(), (long long)n, 0, e->stream));
+        HIP_TRY(hipMemsetAsync(flag.p, 0, 4, e->stream));
+        const int axes[3] = {2, 0, 1};
+        for (int ai = 0; ai < (three ? 3 : 1); ++ai) {
+            const int A = axes[ai];
+            CrossArgs c{};
+            c.axis = A;
+            c.winding = kind == NLOS_VOXEL_WINDING;
+            c.lo[0] = e->cfg.x_min; c.lo[1] = e->cfg.y_min; c.lo[2] = e->cfg.z_min;
+            c.dx = e->cfg.dx;
+            c.n[0] = e->g.nx; c.n[1] = e->g.ny; c.n[2] = e->g.nz;
+            // irrational fractions of 1e-7 dx keep rays off lattice-placed edges and vertices
+            c.off_b = 1e-7 * e->cfg.dx * 0.6180339887498949;
+            c.off_c = 1e-7 * e->cfg.dx * 0.4142135623730950;
+            c.votes = votes.as<uint8_t>();
+            c.overflow = flag.as<int>();
+            HostGeometry::AxisBins ab;
+            if (v.n_tri > 0) {
+                ab = geo.axis_bins(A);
+                HIP_TRY(hipStreamSynchronize(e->stream));   // the previous axis's bins are in use
+                HIP_TRY(bstart.reserve(ab.start.size() * 4));
+                HIP_TRY(blist.reserve(std::max<size_t>(ab.list.size(), 1) * 4));
+                HIP_TRY(hipMemcpy(bstart.p, ab.start.data(), ab.start.size() * 4, hipMemcpyHostToDevice));
+                if (!ab.list.empty()) HIP_TRY(hipMemcpy(blist.p, ab.list.data(), ab.list.size() * 4, hipMemcpyHostToDevice));
+                c.tri = e->vox_tri.as<double>();
+                c.bin_start = bstart.as<int>();
+                c.bin_tris = blist.as<int>();
+                c.rows = 200; c.cols = 200;
+                c.bmin = ab.bmin; c.bmax = ab.bmax; c.cmin = ab.cmin; c.cmax = ab.cmax;
+                c.cell_b = ab.cell_b; c.cell_c = ab.cell_c;
+            }
+            HIP_TRY(launch_crossing_votes(c, e->stream));
+        }
+        HIP_TRY(launch_mask_from_votes(v, e->g, votes.as<uint8_t>(), three ? 2 : 1, e->mask.as<uint8_t>(), e->stream));
+        int over = 0;
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        HIP_TRY(hipMemcpy(&over, flag.p, 4, hipMemcpyDeviceToHost));
+        votes.release(); flag.release(); bstart.release(); blist.release();
+        if (over) return fail(NLOS_ERR_UNSUPPORTED, "a grid line crosses the mesh more than 256 times");
+    }
     HIP_TRY(hipStreamSynchronize(e->stream));
     int rc = pull_host_mask(e);
     if (rc) return rc;
@@ -911,19 +1035,27 @@ NLOS_API int nlos_voxelize_stl(nlos_engine *e, const void *stl, size_t len, doub
     return NLOS_OK;
 }
 
+}  // namespace
+
+NLOS_API int nlos_voxelize_stl(nlos_engine *e, const void *stl, size_t len, double scale, double sx,
+                               double sy, double sz) {
+    if (int rc = ready(e)) return rc;
+    if (e->step != 0) return fail(NLOS_ERR_STATE, "geometry can only change before the first step");
+    HostGeometry geo;
+    const bool ok = stl && geo.parse((const unsigned char *)stl, len, scale, sx, sy, sz);
+    if (!ok) geo = HostGeometry();
+    return voxelize_geometry(e, geo, ok);
+}
+
 NLOS_API int nlos_load_stl(nlos_engine *e, const char *path, double scale, double sx, double sy, double sz) {
     if (int rc = ready(e)) return rc;
-    std::vector<char> buf;
-    if (path) {
-        std::ifstream f(path, std::ios::binary);
-        if (f) buf.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
-    }
-    if (buf.empty()) {
-        int rc = nlos_voxelize_stl(e, nullptr, 0, scale, sx, sy, sz);
-        if (rc != NLOS_OK && rc != NLOS_ERR_IO) return rc;
-        return fail(NLOS_ERR_IO, "STL not loaded (%s), using empty geometry", path ? path : "null");
-    }
-    return nlos_voxelize_stl(e, buf.data(), buf.size(), scale, sx, sy, sz);
+    if (e->step != 0) return fail(NLOS_ERR_STATE, "geometry can only change before the first step");
+    HostGeometry geo;
+    const bool ok = path && geo.parse_file(path, scale, sx, sy, sz);
+    if (!ok) geo = HostGeometry();
+    const int rc = voxelize_geometry(e, geo, ok);
+    if (rc == NLOS_ERR_IO) return fail(NLOS_ERR_IO, "STL not loaded (%s), using empty geometry", path ? path : "null");
+    return rc;
 }
 
 NLOS_API int nlos_get_ground_heights(nlos_engine *e, double *out) {

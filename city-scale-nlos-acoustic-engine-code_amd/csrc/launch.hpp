@@ -86,4 +86,26 @@ hipError_t launch_column_heights(const VoxelArgs &v, const GridDesc &g, double *
 hipError_t launch_mask_from_heights(const VoxelArgs &v, const GridDesc &g, const double *gh,
                                     uint8_t *mask, hipStream_t s);
 
+// Crossing voxeliser (nlos_voxelizer PARITY / WINDING): rays along axis a through the cell
+// centres of every interior grid line; the triangles come binned over the two following
+// axes (b, c) = (a+1, a+2) mod 3, a rows x cols grid over the mesh's (b, c) box.
+struct CrossArgs {
+    const double *tri;         // Triangle13: v1, v2, v3 at [0, 9)
+    const int *bin_start;      // CSR over rows*cols bins
+    const int *bin_tris;
+    int rows, cols;
+    double bmin, bmax, cmin, cmax, cell_b, cell_c;
+    int axis;                  // ray axis a: 0 x, 1 y, 2 z
+    int winding;               // 1: signed crossings, 0: crossing parity
+    double lo[3], dx;          // domain minimum corner and spacing
+    int n[3];                  // interior cells per axis
+    double off_b, off_c;       // ray offsets from the cell-centre lattice
+    uint8_t *votes;            // interior cells, x fastest: += 1 where solid
+    int *overflow;             // set when a line has more crossings than the kernel holds
+};
+hipError_t launch_crossing_votes(const CrossArgs &a, hipStream_t s);
+// Mask from votes: solid where votes >= need (same halo/sky/ground rules as from heights).
+hipError_t launch_mask_from_votes(const VoxelArgs &v, const GridDesc &g, const uint8_t *votes, int need,
+                                  uint8_t *mask, hipStream_t s);
+
 }  // namespace nlos

@@ -99,6 +99,97 @@ hipError_t launch_mask_from_heights(const VoxelArgs &v, const GridDesc &g, const
     return hipGetLastError();
 }
 
+// ---- crossing voxeliser (no counterpart in the reference) -------------------------------
+// One thread per grid line along axis a (interior cells only).  The line (b, c) =
+// (cell centre + offset) meets a triangle where its (b, c) projection holds the point; the
+// crossing's a-coordinate comes from the barycentric weights, and its sign from the a
+// component of the triangle normal (v2-v1) x (v3-v1), which in the cyclic frame (a, b, c)
+// is the signed doubled area of the projection: negative means the ray enters the solid.
+constexpr int MAX_CROSS = 256;   // crossings one line can hold (private memory)
+
+__global__ __launch_bounds__(256) void k_crossing_votes(CrossArgs a) {
+    const int A = a.axis, B = (A + 1) % 3, Cc = (A + 2) % 3;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long lines = (long long)a.n[B] * a.n[Cc];
+    if (t >= lines) return;
+    const int ib = (int)(t % a.n[B]), ic = (int)(t / a.n[B]);
+    const double pb = a.lo[B] + (ib + 0.5) * a.dx + a.off_b;
+    const double pc = a.lo[Cc] + (ic + 0.5) * a.dx + a.off_c;
+    double hz[MAX_CROSS];
+    int hs[MAX_CROSS];
+    int nh = 0;
+    if (a.tri && pb >= a.bmin && pb <= a.bmax && pc >= a.cmin && pc <= a.cmax) {
+        const int r = min(max((int)((pb - a.bmin) / a.cell_b), 0), a.rows - 1);
+        const int q = min(max((int)((pc - a.cmin) / a.cell_c), 0), a.cols - 1);
+        const int bin = r * a.cols + q;
+        for (int n = a.bin_start[bin]; n < a.bin_start[bin + 1]; ++n) {
+            const double *tr = a.tri + (long long)a.bin_tris[n] * 13;
+            const double b1 = tr[B], b2 = tr[3 + B], b3 = tr[6 + B];
+            const double c1 = tr[Cc], c2 = tr[3 + Cc], c3 = tr[6 + Cc];
+            const double d = (b2 - b1) * (c3 - c1) - (b3 - b1) * (c2 - c1);
+            if (d == 0.0) continue;   // parallel to the ray
+            const double w1 = ((b2 - pb) * (c3 - pc) - (b3 - pb) * (c2 - pc)) / d;
+            const double w2 = ((b3 - pb) * (c1 - pc) - (b1 - pb) * (c3 - pc)) / d;
+            const double w3 = 1.0 - w1 - w2;
+            if (w1 < 0.0 || w2 < 0.0 || w3 < 0.0) continue;
+            if (nh == MAX_CROSS) { atomicOr(a.overflow, 1); break; }
+            const double z = w1 * tr[A] + w2 * tr[3 + A] + w3 * tr[6 + A];
+            const int sg = d < 0.0 ? 1 : -1;
+            // insertion into the sorted list
+            int k = nh++;
+            while (k > 0 && hz[k - 1] > z) { hz[k] = hz[k - 1]; hs[k] = hs[k - 1]; --k; }
+            hz[k] = z;
+            hs[k] = sg;
+        }
+    }
+    int h = 0, acc = 0, cnt = 0;
+    for (int s = 0; s < a.n[A]; ++s) {
+        const double z = a.lo[A] + (s + 0.5) * a.dx;
+        while (h < nh && hz[h] < z) { acc += hs[h]; ++cnt; ++h; }
+        const bool inside = a.winding ? acc != 0 : (cnt & 1) != 0;
+        if (!inside) continue;
+        int idx[3];
+        idx[A] = s; idx[B] = ib; idx[Cc] = ic;
+        a.votes[(long long)idx[0] + (long long)a.n[0] * ((long long)idx[1] + (long long)a.n[1] * idx[2])] += 1;
+    }
+}
+
+hipError_t launch_crossing_votes(const CrossArgs &a, hipStream_t s) {
+    const long long lines = (long long)a.n[(a.axis + 1) % 3] * a.n[(a.axis + 2) % 3];
+    if (lines <= 0) return hipSuccess;
+    k_crossing_votes<<<(unsigned)((lines + 255) / 256), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// The rules of k_mask_from_heights with "z < column height" replaced by votes >= need.
+__global__ __launch_bounds__(256) void k_mask_from_votes(VoxelArgs v, GridDesc g, const uint8_t *votes, int need,
+                                                        uint8_t *mask) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    if (t >= total) return;
+    const int i = (int)(t % (g.nx + 2));
+    const int j = (int)((t / (g.nx + 2)) % (g.ny + 2));
+    const int k = (int)(t / ((long long)(g.nx + 2) * (g.ny + 2)));
+    const long long c = cell_index(g, i, j, k);
+    uint8_t m;
+    if (k == 0) m = v.ground ? MK_SOLID : 0;
+    else if (k == g.nz + 1 || i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1) m = 0;
+    else {
+        const double z = v.z_min + ((k - 1) + 0.5) * v.dx;
+        const uint8_t vt = votes[(long long)(i - 1) + (long long)g.nx * ((long long)(j - 1) + (long long)g.ny * (k - 1))];
+        m = (v.use_geometry && vt >= need) ? MK_SOLID : 0;
+        if (z > v.sky_z) m = 0;
+    }
+    mask[c] = m;
+}
+
+hipError_t launch_mask_from_votes(const VoxelArgs &v, const GridDesc &g, const uint8_t *votes, int need,
+                                  uint8_t *mask, hipStream_t s) {
+    const long long total = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    k_mask_from_votes<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(v, g, votes, need, mask);
+    return hipGetLastError();
+}
+
 // NEAR bit on fluid interior cells with a solid neighbour among the 18 links; keeps the
 // SOLID and PROBE bits.
 __global__ __launch_bounds__(256) void k_mask_flags(uint8_t *mask, GridDesc g) {

@@ -68,6 +68,7 @@ public:
     bool ground = true;
     double wall_reflect = 1.0;
     int precision = NLOS_F64;
+    int voxelizer = NLOS_VOXEL_HEIGHT;   // the reference's height field; see nlos_voxelizer
     int waveform = NLOS_WAVE_GAUSS_PULSE;
     int injection = NLOS_INJECT_GAUSS5;
     bool write_frames = true;
@@ -100,6 +101,7 @@ public:
         c.use_geometry = use_geometry ? 1 : 0;
         c.output_world_coords = output_world_coords ? 1 : 0;
         c.precision = precision;
+        c.voxelizer = voxelizer;
         check(nlos_create(&c, &eng_));
         nlos_info info;
         check(nlos_get_info(eng_, &info));

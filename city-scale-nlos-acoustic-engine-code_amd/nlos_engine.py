@@ -21,6 +21,7 @@ LIB_PATH = os.environ.get("NLOS_ENGINE_LIB") or os.path.join(HERE, "libnlos_engi
 
 F64, F32, F16S, B16 = 0, 1, 2, 3
 VTK_ASCII, VTK_BINARY = 0, 1
+VOXEL_HEIGHT, VOXEL_PARITY, VOXEL_WINDING, VOXEL_3AXIS = 0, 1, 2, 4
 PRECISIONS = {"f64": F64, "f32": F32, "f16s": F16S, "b16": B16}
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = 1, 2, 4, 8, 16, 32
 FACES_REFERENCE = FACE_XM | FACE_XP | FACE_YM | FACE_YP | FACE_ZP
@@ -45,7 +46,7 @@ class Config(C.Structure):
                 ("sponge_faces", C.c_int32), ("nscbc_faces", C.c_int32), ("ground", C.c_int32),
                 ("use_geometry", C.c_int32), ("output_world_coords", C.c_int32),
                 ("precision", C.c_int32), ("f16_scale_log2", C.c_int32),
-                ("stream_pattern", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("stream_pattern", C.c_int32), ("voxelizer", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Source(C.Structure):

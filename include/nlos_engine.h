@@ -85,8 +85,29 @@ typedef struct nlos_config {
     int32_t precision;         /* nlos_precision */
     int32_t f16_scale_log2;    /* NLOS_F16S: stored value = g * 2^k; 0 selects the default 24 */
     int32_t stream_pattern;    /* nlos_pattern */
-    int32_t reserved[4];
+    int32_t voxelizer;         /* nlos_voxelizer; 0 is the reference's height field */
+    int32_t reserved[3];
 } nlos_config;
+
+/* How an STL mesh becomes solid cells.
+ * NLOS_VOXEL_HEIGHT is the reference: the top hit of a vertical ray per column, every cell
+ * centre below it solid (05:144-192, 469-489).  It fills whatever lies under an overhang.
+ * The others decide per cell centre from the mesh crossings of rays through it (no
+ * counterpart in the reference):
+ *   NLOS_VOXEL_PARITY  - solid where the crossing count from below is odd (closed meshes;
+ *                        orientation is not used, overlapping shells cancel);
+ *   NLOS_VOXEL_WINDING - solid where the signed crossing count (entering through a face
+ *                        whose normal, from the vertex order, points against the ray: +1,
+ *                        leaving: -1) is non-zero (outward-oriented meshes; overlapping
+ *                        shells give their union, inward shells carve cavities).
+ * Adding NLOS_VOXEL_3AXIS casts rays along x, y and z and keeps a cell solid when at least
+ * two of them say so, which tolerates holes in the mesh.  Rays sit 1e-7 dx (times an
+ * irrational factor) off the cell-centre lattice so that they miss mesh edges placed on it.
+ * The mask rules around the mesh (sky_z, halo, ground) are the reference's in every mode;
+ * nlos_get_ground_heights returns the column top hits in every mode. */
+typedef enum nlos_voxelizer {
+    NLOS_VOXEL_HEIGHT = 0, NLOS_VOXEL_PARITY = 1, NLOS_VOXEL_WINDING = 2, NLOS_VOXEL_3AXIS = 4
+} nlos_voxelizer;
 
 /* How populations stream.  Both give identical results; they differ in memory and speed.
  * NLOS_PATTERN_INPLACE keeps one array (A-A: alternating own-cell and neighbour sweeps);
