@@ -468,6 +468,83 @@ def bench_frames(args) -> None:
     print(json.dumps(out), flush=True)
 
 
+def bench_legacy04(args) -> None:
+    """The legacy family-A step (NLOS_FAMILY_A, 01/04 step_pull semantics, FP64) on
+    04_hpc_scalability.cpp's benchmark geometry: +-13 m box at dx = 0.125 (208^3 cells),
+    tau 0.51, a solid cylinder of radius 4 along z, 04's radial sponge (beyond r = 10 over 3,
+    strength 0.5), z periodic.  04's timed loop: 100 warm-up steps, 2000 timed, MLUPS over
+    all nodes (04:360-404).  04's moving source is left out (family-A sources are not
+    built); the run starts from a small density pulse instead.  CPU baseline: the oracle's
+    family-A restatement (OpenMP) on a bounded number of steps of the same grid."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import math
+    import family_a as fa
+    steps = args.steps if args.steps != 200 else 2000
+    warmup = args.warmup if args.warmup != 20 else 100
+    cfg = fa.config(x=(-13.0, 13.0), y=(-13.0, 13.0), z=(-13.0, 13.0), dx=0.125, tau=0.51)
+    nx, ny, nz = fa.dims(cfg)
+    solid, rho = fa.cylinder_and_pulse(cfg, radius=4.0, centre=(0.0, -6.0), width=2.0, amp=0.01)
+    sg = np.zeros((ny, nx))
+    for j in range(ny):
+        for i in range(nx):
+            x = cfg.x_min + float(i) * cfg.dx + 0.5 * cfg.dx          # 04:187-189
+            y = cfg.y_min + float(j) * cfg.dx + 0.5 * cfg.dx
+            r = math.sqrt(x * x + y * y)
+            if r > 10.0:
+                ratio = min(1.0, (r - 10.0) / 3.0)
+                sg[j, i] = 0.5 * ratio * ratio
+    sigma = np.broadcast_to(sg, (nz, ny, nx)).copy()
+    e = nlos.Engine(cfg)
+    e.set_mask(solid)
+    e.set_density(rho)
+    e.set_sponge(sigma)
+    e.set_probes([(0.0, 9.0, 0.0)])
+    log(f"legacy04: {nx}x{ny}x{nz}, warmup {warmup}, timed {steps}")
+    e.step(warmup)
+    e.synchronize()
+    e.timing(True)
+    e.timing_reset()
+    t = time.perf_counter()
+    e.step(steps)
+    e.synchronize()
+    wall = time.perf_counter() - t
+    tm = e.timing(False)
+    cells = nx * ny * nz
+    n_solid = int(solid.sum())
+    mlups = cells * steps / wall / 1e6
+    # per fluid cell and step: collide 19 loads + 19 stores, stream 19 loads + 19 stores,
+    # FP64, plus the mask byte and the sponge value; solid cells 1 B
+    bpu = 76 * 8 + 1 + 8
+    alg = (cells - n_solid) * bpu + n_solid
+    out = {"metric": "MLUPS, legacy family-A step (04_hpc_scalability geometry 208^3, FP64)",
+           "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"04 geometry: {nx}x{ny}x{nz} cells, cylinder r=4, radial sponge, "
+                                  "z periodic, initial pulse instead of 04's source"},
+           "sweep_ms": round(tm.avg_ms, 4),
+           "roofline": {"bound": "hbm", "bytes_per_update": bpu,
+                        "achieved": round(alg / (wall / steps) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(alg / (wall / steps) / 1e9 / HBM_PEAK_GBS, 4)}}
+    if not args.no_cpu:
+        from oracle_ffi import OracleA
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = OracleA(cfg)
+        o.set_mask(solid)
+        o.set_density(rho)
+        o.set_sponge(sigma)
+        o.run(1)
+        n_cpu = 40                 # about 10-20 s on the host cores
+        t = time.perf_counter()
+        o.run(n_cpu)
+        cw = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(cells * n_cpu / cw / 1e6, 2), "unit": "MLUPS", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle family-A restatement (01's loop, OpenMP), {n_cpu} steps of "
+                                         f"the same {nx}x{ny}x{nz} grid in {cw:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -484,14 +561,14 @@ def main() -> None:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 2 GiB device copy kernel (PMC byte calibration)")
     ap.add_argument("--workload", default="city",
-                    choices=["city", "scatter", "city2048", "batch", "adjoint", "frames"],
+                    choices=["city", "scatter", "city2048", "batch", "adjoint", "frames", "legacy04"],
                     help="city: the headline moving-source city step (default); scatter: the "
                          "128^3 rigid-cube case, FP64 against the oracle; city2048: the dx=0.5 m "
                          "precision sweep on 2048x2048x512 with its 512x512x256 error sub-box; "
                          "batch: 256 small FP32 scenes advanced together; adjoint: source-"
                          "localisation gradient in a 256x256x128 canyon against finite differences; "
                          "frames: a city run writing binary VTK frames synchronously and off the "
-                         "step stream")
+                         "step stream; legacy04: the family-A step of 01/04 on 04's benchmark geometry")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -509,6 +586,8 @@ def main() -> None:
         return bench_adjoint(args)
     if args.workload == "frames":
         return bench_frames(args)
+    if args.workload == "legacy04":
+        return bench_legacy04(args)
 
     wl = workloads.CityWorkload(nx=args.nx, ny=args.ny, nz=args.nz)
     t = time.perf_counter()

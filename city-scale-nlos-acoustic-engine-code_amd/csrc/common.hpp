@@ -167,6 +167,22 @@ struct AdjArgs {
     const double *dl_dp;                  // dL/d(rho-1) of this step per unique probe cell, or null
 };
 
+// One step of the legacy family-A mode (legacy_a.hip).  f holds the populations in
+// ordering A and is updated in place; post receives the post-collision values.
+struct AStepArgs {
+    double *f;
+    double *post;
+    const uint8_t *mask;
+    const double *sigma;                  // sponge per interior cell, x fastest
+    GridDesc g;
+    double tau;
+    const long long *probe_cells;         // sorted unique probe cells
+    int n_probe_cells;
+    double *probe_row;                    // rho - 1 after this step per unique probe cell
+    double *field;                        // rho per interior cell (capture steps) or null
+    int *flag;                            // bit 0: non-finite density seen
+};
+
 // Source entries of several scenes over a run of steps, read at the step the device counter
 // names: entries step_off[t] .. step_off[t+1] of step t = *ctr - *ctr_base.
 struct SourceMulti {

@@ -259,6 +259,12 @@ struct nlos_engine {
     std::vector<long long> sched_cells;
     std::vector<double> sched_adds;
 
+    // legacy family-A mode (legacy_a.hip)
+    int family = NLOS_FAMILY_B;
+    DevBuf sigma_a;                   // sponge per interior cell (FP64)
+    DevBuf field_a;                   // rho per interior cell of the last capture step (FP64)
+    std::vector<double> rho0;         // density set before the first step (t = 0 probe line)
+
     // field dump off the step stream (nlos_dump_field_async)
     hipStream_t copy_stream = nullptr;
     hipEvent_t dump_ready = nullptr, dump_copied = nullptr;
@@ -336,11 +342,40 @@ int upload_sponge(nlos_engine *e) {
     return NLOS_OK;
 }
 
+// Family A's default sponge (01:207-225): on the x and y faces, d = how far the cell
+// centre lies inside the layer of thickness T, sigma = S*(max(dx, dy)/T)^2 when positive.
+int upload_sponge_a(nlos_engine *e) {
+    const nlos_config &c = e->cfg;
+    const double T = c.sponge_width, S = c.sponge_strength;
+    std::vector<double> sg((size_t)e->n_interior(), 0.0);
+    size_t n = 0;
+    for (int k = 1; k <= e->g.nz; ++k)
+        for (int j = 1; j <= e->g.ny; ++j)
+            for (int i = 1; i <= e->g.nx; ++i, ++n) {
+                const double x = c.x_min + ((i - 1) + 0.5) * c.dx;
+                const double y = c.y_min + ((j - 1) + 0.5) * c.dx;
+                double dxs = 0.0, dys = 0.0;
+                if (x < c.x_min + T) dxs = (c.x_min + T) - x;
+                else if (x > c.x_max - T) dxs = x - (c.x_max - T);
+                if (y < c.y_min + T) dys = (c.y_min + T) - y;
+                else if (y > c.y_max - T) dys = y - (c.y_max - T);
+                const double dm = std::max(dxs, dys);
+                if (dm > 0.0) {
+                    const double r = dm / T;
+                    sg[n] = S * r * r;
+                }
+            }
+    HIP_TRY(e->sigma_a.reserve(sg.size() * 8));
+    HIP_TRY(hipMemcpy(e->sigma_a.p, sg.data(), sg.size() * 8, hipMemcpyHostToDevice));
+    return NLOS_OK;
+}
+
 // Probe bits, NEAR bits, tracked solid cells on open faces; state reset to rest.
 int finalize_mask(nlos_engine *e) {
     GridDesc &g = e->g;
     e->fb_valid = false;
     e->walls_valid = false;
+    if (e->family == NLOS_FAMILY_A) HIP_TRY(launch_mask_periodic_z(e->mask.as<uint8_t>(), g, e->stream));
     HIP_TRY(launch_mask_flags(e->mask.as<uint8_t>(), g, e->stream));
     if (e->wall_bits.p) HIP_TRY(launch_wall_bits(e->mask.as<uint8_t>(), g, e->wall_bits.as<uint32_t>(), e->stream));
     for (long long c : e->probe_ucells) {
@@ -741,6 +776,46 @@ int launch_sweep(nlos_engine *e, bool capture) {
     return NLOS_OK;
 }
 
+// One family-A step (legacy_a.hip); the probe row of this step is recorded.
+int launch_a(nlos_engine *e, bool capture) {
+    if (capture) {
+        if (e->dump_pending) HIP_TRY(hipStreamWaitEvent(e->stream, e->dump_copied, 0));
+        HIP_TRY(e->field_a.reserve((size_t)e->n_interior() * 8));
+    }
+    HIP_TRY(launch_a_halo(static_cast<double *>(e->pop.p), e->g, e->stream));
+    AStepArgs a{};
+    a.f = static_cast<double *>(e->pop.p);
+    a.post = static_cast<double *>(e->pop2.p);
+    a.mask = e->mask.as<uint8_t>();
+    a.sigma = e->sigma_a.as<double>();
+    a.g = e->g;
+    a.tau = e->cfg.tau;
+    a.n_probe_cells = (int)e->probe_ucells.size();
+    a.probe_cells = a.n_probe_cells ? e->probe_cells.as<long long>() : nullptr;
+    a.probe_row = a.n_probe_cells ? e->hist.as<double>() + (size_t)e->step * a.n_probe_cells : nullptr;
+    a.field = capture ? e->field_a.as<double>() : nullptr;
+    a.flag = e->flag.as<int>();
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e->timing) {
+        if (e->ev_used + 2 > e->ev.size()) {
+            for (int q = 0; q < 64; ++q) {
+                hipEvent_t x;
+                HIP_TRY(hipEventCreate(&x));
+                e->ev.push_back(x);
+            }
+        }
+        t0 = e->ev[e->ev_used];
+        t1 = e->ev[e->ev_used + 1];
+        e->ev_used += 2;
+        HIP_TRY(hipEventRecord(t0, e->stream));
+    }
+    HIP_TRY(launch_a_sweep(a, e->stream));
+    if (e->timing) HIP_TRY(hipEventRecord(t1, e->stream));
+    if (capture) e->field_valid = true;
+    ++e->step;
+    return NLOS_OK;
+}
+
 int collect_timing(nlos_engine *e) {
     if (e->ev_used == 0) return NLOS_OK;
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -817,16 +892,29 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     if (cfg->precision < NLOS_F64 || cfg->precision > NLOS_B16) return fail(NLOS_ERR_ARG, "bad precision");
     if (cfg->precision == NLOS_B16 && cfg->stream_pattern == NLOS_PATTERN_INPLACE)
         return fail(NLOS_ERR_UNSUPPORTED, "NLOS_B16 needs the two-array stream pattern");
+    if (cfg->family != NLOS_FAMILY_B && cfg->family != NLOS_FAMILY_A) return fail(NLOS_ERR_ARG, "bad family");
+    const bool fam_a = cfg->family == NLOS_FAMILY_A;
+    if (fam_a && cfg->precision != NLOS_F64) return fail(NLOS_ERR_UNSUPPORTED, "NLOS_FAMILY_A runs in NLOS_F64");
+    if (fam_a && cfg->stream_pattern == NLOS_PATTERN_INPLACE)
+        return fail(NLOS_ERR_UNSUPPORTED, "NLOS_FAMILY_A needs the two-array stream pattern");
     if (!(cfg->wall_reflect >= 0.0 && cfg->wall_reflect <= 1.0)) return fail(NLOS_ERR_ARG, "wall_reflect outside [0,1]");
     const int NX = (int)((cfg->x_max - cfg->x_min) / cfg->dx + 0.5);   // 05:332-334
     const int NY = (int)((cfg->y_max - cfg->y_min) / cfg->dx + 0.5);
-    const int NZ = (int)((cfg->z_max - cfg->z_min) / cfg->dx + 0.5);
+    int NZ = (int)((cfg->z_max - cfg->z_min) / cfg->dx + 0.5);
+    if (fam_a && NZ < 3) NZ = 3;                                        // 01:100
     if (NX < 1 || NY < 1 || NZ < 1) return fail(NLOS_ERR_ARG, "empty grid %d x %d x %d", NX, NY, NZ);
     if ((long long)(NY + 2) * (NZ) > 0x7fffffffLL / 64) return fail(NLOS_ERR_ARG, "grid too large");
 
     nlos_engine *e = new nlos_engine();
     e->cfg = *cfg;
     e->mode = cfg->precision;
+    e->family = cfg->family;
+    if (fam_a) {
+        // family A has no ground halo and no NSCBC; its halos are set by the step itself
+        e->cfg.ground = 0;
+        e->cfg.nscbc_faces = 0;
+        e->cfg.stream_pattern = NLOS_PATTERN_TWO_ARRAY;
+    }
     e->g.nx = NX; e->g.ny = NY; e->g.nz = NZ;
     e->g.px = ((NX + 2 + 63) / 64) * 64;
     e->g.py = NY + 2; e->g.pz = NZ + 2;
@@ -834,6 +922,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     e->g.origin = 63;
     e->g.ncell = ((e->g.origin + e->g.plane * e->g.pz + 63) / 64) * 64;
     e->dt = (CS * cfg->dx) / cfg->c_phys;     // 05:200-203
+    if (fam_a) e->dt = std::sqrt(1.0 / 3.0) * cfg->dx / cfg->c_phys;     // 01:37-41
     e->omega = 1.0 / cfg->tau;                // 05:908
     e->s_bytes = e->mode == M_F64 ? 8 : (e->mode == M_F32 ? 4 : 2);   // per population
     e->c_bytes = e->mode == M_F64 ? 8 : 4;
@@ -851,7 +940,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)));
     const size_t popb = pop_bytes(e);
-    e->pattern = cfg->stream_pattern;
+    e->pattern = e->cfg.stream_pattern;
     if (e->pattern == NLOS_PATTERN_AUTO) {
         size_t freeb = 0, totalb = 0;
         if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = 0;
@@ -882,6 +971,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
                 return bail(fail(NLOS_ERR_NOMEM, "cannot allocate x-face buffers"));
     int rc;
     if ((rc = upload_sponge(e)) != NLOS_OK) return bail(rc);
+    if (fam_a && (rc = upload_sponge_a(e)) != NLOS_OK) return bail(rc);
     if (hipMemsetAsync(e->flag.p, 0, sizeof(int), e->stream) != hipSuccess)
         return bail(fail(NLOS_ERR_HIP, "memset flag"));
     if (by_mode(e->mode, [&](auto L) { return decltype(L)::fill_rest(e->pop.p, e->g, e->stream); }) != hipSuccess)
@@ -918,7 +1008,8 @@ NLOS_API int nlos_destroy(nlos_engine *e) {
                       &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
                       &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
                       &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
-                      &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1], &e->wall_bits};
+                      &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1], &e->wall_bits,
+                      &e->sigma_a, &e->field_a};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : e->ck) b.release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
@@ -942,7 +1033,8 @@ NLOS_API int nlos_get_info(const nlos_engine *e, nlos_info *info) {
     size_t tot = 0;
     const DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                             &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
-                            &e->src_adds, &e->gh, &e->fb[0], &e->fb[1], &e->wall_bits};
+                            &e->src_adds, &e->gh, &e->fb[0], &e->fb[1], &e->wall_bits, &e->sigma_a,
+                            &e->field_a};
     for (const DevBuf *b : bufs) tot += b->bytes;
     info->device_bytes = tot;
     return NLOS_OK;
@@ -1095,11 +1187,20 @@ NLOS_API int nlos_get_mask(nlos_engine *e, uint8_t *m) {
 }
 
 // ---- initial state, source, probes ----------------------------------------------------------
+NLOS_API int nlos_set_sponge(nlos_engine *e, const double *sigma) {
+    if (int rc = ready(e)) return rc;
+    if (e->family != NLOS_FAMILY_A) return fail(NLOS_ERR_UNSUPPORTED, "per-cell sponges are a NLOS_FAMILY_A feature");
+    if (!sigma) return fail(NLOS_ERR_ARG, "null sponge");
+    HIP_TRY(hipMemcpy(e->sigma_a.p, sigma, (size_t)e->n_interior() * 8, hipMemcpyHostToDevice));
+    return NLOS_OK;
+}
+
 NLOS_API int nlos_set_density(nlos_engine *e, const double *rho) {
     if (int rc = ready(e)) return rc;
     if (!rho) return fail(NLOS_ERR_ARG, "null density");
     if (e->step != 0) return fail(NLOS_ERR_STATE, "initial density can only be set before the first step");
     const size_t n = (size_t)e->n_interior();
+    if (e->family == NLOS_FAMILY_A) e->rho0.assign(rho, rho + n);   // 01's t = 0 probe values
     HIP_TRY(e->rho_tmp.reserve(n * 8));
     HIP_TRY(hipMemcpy(e->rho_tmp.p, rho, n * 8, hipMemcpyHostToDevice));
     HIP_TRY(by_mode(e->mode, [&](auto L) {
@@ -1115,6 +1216,7 @@ NLOS_API int nlos_set_density(nlos_engine *e, const double *rho) {
 NLOS_API int nlos_set_source(nlos_engine *e, const nlos_source *s) {
     if (int rc = ready(e)) return rc;
     if (!s) return fail(NLOS_ERR_ARG, "null source");
+    if (e->family == NLOS_FAMILY_A) return fail(NLOS_ERR_UNSUPPORTED, "sources are a NLOS_FAMILY_B feature");
     if (s->waveform < NLOS_WAVE_GAUSS_PULSE || s->waveform > NLOS_WAVE_RICKER) return fail(NLOS_ERR_ARG, "bad waveform");
     if (s->injection != NLOS_INJECT_GAUSS5 && s->injection != NLOS_INJECT_TRILINEAR) return fail(NLOS_ERR_ARG, "bad injection");
     if (s->injection == NLOS_INJECT_GAUSS5 && (s->stencil_radius < 0 || s->stencil_radius > 3))
@@ -1136,9 +1238,15 @@ NLOS_API int nlos_set_probes(nlos_engine *e, const double *xyz, int32_t n, uint8
     for (int p = 0; p < n; ++p) {
         ProbeDesc d{};
         d.x = xyz[3 * p]; d.y = xyz[3 * p + 1]; d.z = xyz[3 * p + 2];
-        d.gi = (int)std::floor((d.x - e->cfg.x_min) / e->cfg.dx);     // 05:417-419
-        d.gj = (int)std::floor((d.y - e->cfg.y_min) / e->cfg.dx);
-        d.gk = (int)std::floor((d.z - e->cfg.z_min) / e->cfg.dx);
+        if (e->family == NLOS_FAMILY_A) {
+            d.gi = (int)((d.x - e->cfg.x_min) / e->cfg.dx - 0.5);     // 01:452-454
+            d.gj = (int)((d.y - e->cfg.y_min) / e->cfg.dx - 0.5);
+            d.gk = (int)((d.z - e->cfg.z_min) / e->cfg.dx - 0.5);
+        } else {
+            d.gi = (int)std::floor((d.x - e->cfg.x_min) / e->cfg.dx);     // 05:417-419
+            d.gj = (int)std::floor((d.y - e->cfg.y_min) / e->cfg.dx);
+            d.gk = (int)std::floor((d.z - e->cfg.z_min) / e->cfg.dx);
+        }
         if (d.gi < 0 || d.gi >= e->g.nx || d.gj < 0 || d.gj >= e->g.ny || d.gk < 0 || d.gk >= e->g.nz)
             return fail(NLOS_ERR_ARG, "probe %d at (%g,%g,%g) lies outside the grid", p, d.x, d.y, d.z);
         d.in_solid = e->solid_interior(d.gi, d.gj, d.gk);
@@ -1174,6 +1282,7 @@ NLOS_API int nlos_set_probes(nlos_engine *e, const double *xyz, int32_t n, uint8
 NLOS_API int nlos_apply_source(nlos_engine *e, double t) {
     if (int rc = ready(e)) return rc;
     if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
+    if (e->family == NLOS_FAMILY_A) return fail(NLOS_ERR_UNSUPPORTED, "sources are a NLOS_FAMILY_B feature");
     e->sched_cells.clear();
     e->sched_adds.clear();
     schedule_step(e, t);
@@ -1185,6 +1294,10 @@ NLOS_API int nlos_apply_source(nlos_engine *e, double t) {
 NLOS_API int nlos_exchange_halos(nlos_engine *e) {
     if (int rc = ready(e)) return rc;
     if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
+    if (e->family == NLOS_FAMILY_A) {
+        HIP_TRY(launch_a_halo(static_cast<double *>(e->pop.p), e->g, e->stream));
+        return NLOS_OK;
+    }
     return launch_halos(e);
 }
 
@@ -1193,6 +1306,11 @@ NLOS_API int nlos_step_collision(nlos_engine *e) {
     if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
     int rc = ensure_hist(e, e->step + 1);
     if (rc) return rc;
+    if (e->family == NLOS_FAMILY_A) {
+        // the halo of exchange_halos is refreshed as part of the step: 01 always runs the two
+        // together (01:526-527), and doing it twice changes nothing
+        return launch_a(e, true);
+    }
     return launch_sweep(e, true);
 }
 
@@ -1203,6 +1321,12 @@ NLOS_API int nlos_step(nlos_engine *e, int64_t n, int32_t flags) {
     if (n == 0) return NLOS_OK;
     int rc = ensure_hist(e, e->step + n);
     if (rc) return rc;
+    if (e->family == NLOS_FAMILY_A) {
+        for (long long s = 0; s < n; ++s)
+            if ((rc = launch_a(e, (flags & NLOS_STEP_CAPTURE_FIELD) && s == n - 1))) return rc;
+        if (e->timing && (rc = collect_timing(e))) return rc;
+        return NLOS_OK;
+    }
     const long long chunk = 4096;
     for (long long base = 0; base < n; base += chunk) {
         const long long m = std::min(chunk, n - base);
@@ -1251,6 +1375,12 @@ NLOS_API int nlos_read_field(nlos_engine *e, float *out) {
     if (!out) return fail(NLOS_ERR_ARG, "null output");
     if (!e->field_valid) return fail(NLOS_ERR_STATE, "no field captured (step with NLOS_STEP_CAPTURE_FIELD)");
     if (int rc = check_flag(e)) return rc;
+    if (e->family == NLOS_FAMILY_A) {
+        std::vector<double> r((size_t)e->n_interior());
+        HIP_TRY(hipMemcpy(r.data(), e->field_a.p, r.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < r.size(); ++q) out[q] = (float)(r[q] - 1.0);
+        return NLOS_OK;
+    }
     HIP_TRY(hipMemcpy(out, e->field.p, (size_t)e->n_interior() * 4, hipMemcpyDeviceToHost));
     return NLOS_OK;
 }
@@ -1314,9 +1444,40 @@ int write_vtk_host(const nlos_engine *e, const char *path, const float *v, int f
     return ok ? NLOS_OK : fail(NLOS_ERR_IO, "write failed: %s", path);
 }
 
+// 01's frame (01:414-436): rho of every interior cell (1 on solids), written as
+// (rho - 1) * 100 with "%.6e", non-finite values as 0.
+int write_vtk_a(nlos_engine *e, const char *path) {
+    if (!e->field_valid) return fail(NLOS_ERR_STATE, "no field captured (step with NLOS_STEP_CAPTURE_FIELD)");
+    if (int rc = check_flag(e)) return rc;
+    const size_t n = (size_t)e->n_interior();
+    std::vector<double> r(n);
+    HIP_TRY(hipMemcpy(r.data(), e->field_a.p, n * 8, hipMemcpyDeviceToHost));
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(NLOS_ERR_IO, "cannot write %s", path);
+    std::vector<char> iobuf((size_t)1 << 22);
+    std::setvbuf(f, iobuf.data(), _IOFBF, iobuf.size());
+    std::fprintf(f, "# vtk DataFile Version 3.0\nLBM Final Corrected\nASCII\nDATASET STRUCTURED_POINTS\n");
+    std::fprintf(f, "DIMENSIONS %d %d %d\n", e->g.nx, e->g.ny, e->g.nz);
+    std::fprintf(f, "ORIGIN %g %g %g\n", e->cfg.x_min, e->cfg.y_min, e->cfg.z_min);
+    std::fprintf(f, "SPACING %g %g %g\n", e->cfg.dx, e->cfg.dx, e->cfg.dx);
+    std::fprintf(f, "POINT_DATA %zu\n", n);
+    std::fprintf(f, "SCALARS Pressure double 1\nLOOKUP_TABLE default\n");
+    for (size_t q = 0; q < n; ++q) {
+        double v = r[q];
+        if (std::isnan(v) || std::isinf(v)) v = 1.0;
+        std::fprintf(f, "%.6e\n", (v - 1.0) * 100.0);
+    }
+    const bool ok = !std::ferror(f);
+    return (std::fclose(f) == 0 && ok) ? NLOS_OK : fail(NLOS_ERR_IO, "write failed: %s", path);
+}
+
 int write_vtk_sync(nlos_engine *e, const char *path, int format) {
     if (int rc = ready(e)) return rc;
     if (!path) return fail(NLOS_ERR_ARG, "null path");
+    if (e->family == NLOS_FAMILY_A) {
+        if (format != NLOS_VTK_ASCII) return fail(NLOS_ERR_UNSUPPORTED, "NLOS_FAMILY_A writes 01's ASCII frames");
+        return write_vtk_a(e, path);
+    }
     std::vector<float> v((size_t)e->n_interior());
     if (int rc = nlos_read_field(e, v.data())) return rc;
     return write_vtk_host(e, path, v.data(), format);
@@ -1346,6 +1507,7 @@ NLOS_API int nlos_dump_field_async(nlos_engine *e, const char *path, int32_t for
     if (int rc = ready(e)) return rc;
     if (!path) return fail(NLOS_ERR_ARG, "null path");
     if (format != NLOS_VTK_ASCII && format != NLOS_VTK_BINARY) return fail(NLOS_ERR_ARG, "bad VTK format %d", format);
+    if (e->family == NLOS_FAMILY_A) return fail(NLOS_ERR_UNSUPPORTED, "NLOS_FAMILY_A frames are written synchronously");
     if (!e->field_valid) return fail(NLOS_ERR_STATE, "no field captured (step with NLOS_STEP_CAPTURE_FIELD)");
     if (int rc = nlos_dump_wait(e)) return rc;
     const size_t bytes = (size_t)e->n_interior() * 4;
@@ -1387,6 +1549,14 @@ NLOS_API int nlos_write_probe_dat(nlos_engine *e, const char *dir, const char *c
         FILE *f = std::fopen(path.c_str(), "w");
         if (!f) return fail(NLOS_ERR_IO, "cannot write %s", path.c_str());
         std::fprintf(f, "# Time Pressure\n");                                        // 01:487
+        if (e->family == NLOS_FAMILY_A) {
+            // 01 records before every step, so its first line is the state at t = 0 (01:509)
+            const ProbeDesc &d = e->probes[p];
+            double v0 = 0.0;
+            if (!d.in_solid && !e->rho0.empty())
+                v0 = e->rho0[(size_t)d.gi + (size_t)e->g.nx * ((size_t)d.gj + (size_t)e->g.ny * d.gk)] - 1.0;
+            std::fprintf(f, "%g %g\n", 0.0, v0 * scale);
+        }
         for (long long s = 0; s < got; ++s)                                          // 01:488-491
             std::fprintf(f, "%g %g\n", (double)(s + 1) * e->dt, buf[(size_t)p * ns + s] * scale);
         if (std::fclose(f) != 0) return fail(NLOS_ERR_IO, "write failed: %s", path.c_str());
@@ -1807,6 +1977,7 @@ NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **
     *out = nullptr;
     for (int i = 0; i < n; ++i) {
         if (cfgs[i].precision != cfgs[0].precision) return fail(NLOS_ERR_ARG, "scenes of a batch share one precision");
+        if (cfgs[i].family != NLOS_FAMILY_B) return fail(NLOS_ERR_UNSUPPORTED, "batches run NLOS_FAMILY_B scenes");
         if (cfgs[i].stream_pattern == NLOS_PATTERN_INPLACE)
             return fail(NLOS_ERR_UNSUPPORTED, "batched scenes use the two-array pattern");
     }
@@ -2020,6 +2191,7 @@ int adjoint_supported(const nlos_engine *e) {
     if (e->cfg.nscbc_faces != 0) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint needs nscbc_faces == 0");
     if (e->mode == M_F16 || e->mode == M_B16)
         return fail(NLOS_ERR_UNSUPPORTED, "the adjoint supports NLOS_F64 and NLOS_F32");
+    if (e->family == NLOS_FAMILY_A) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint is a NLOS_FAMILY_B feature");
     if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch");
     return NLOS_OK;
 }

@@ -59,6 +59,7 @@ struct Launch {
 
 // Mode-independent helpers (voxel.hip).
 hipError_t launch_mask_flags(uint8_t *mask, const GridDesc &g, hipStream_t s);
+hipError_t launch_mask_periodic_z(uint8_t *mask, const GridDesc &g, hipStream_t s);
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, long long n, uint8_t v, hipStream_t s);
 // Per interior cell: bit q set when c - e_q is solid (two-array bounce-back).
@@ -67,6 +68,11 @@ hipError_t launch_wall_bits(const uint8_t *mask, const GridDesc &g, uint32_t *bi
 hipError_t launch_count(long long *ctr, long long by, hipStream_t s);
 // Grid-stride copy of n16 16-byte words.
 hipError_t launch_copy16(const void *a, void *b, long long n16, hipStream_t s);
+
+// Legacy family-A step (legacy_a.hip): the halo of 01's exchange_halos, then collide and
+// stream (01's step_pull).
+hipError_t launch_a_halo(double *f, const GridDesc &g, hipStream_t s);
+hipError_t launch_a_sweep(const AStepArgs &a, hipStream_t s);
 
 // Voxeliser (voxel.hip).
 struct VoxelArgs {

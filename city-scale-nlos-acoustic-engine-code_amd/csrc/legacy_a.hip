@@ -1,0 +1,167 @@
+// Family-A step (01_acoustic_scattering.cpp / 04_hpc_scalability.cpp step_pull semantics)
+// for the engine's legacy mode, FP64 with contraction off so that every value rounds as in
+// the reference.  Four launches per step:
+//   k_a_halo_xy   x and y halo cells at rest (no neighbour rank, 01:234-246)
+//   k_a_halo_z    z halo planes copied periodically, x/y halo columns included (01:283-300)
+//   k_a_collide   post-collision value of every fluid cell, halo included (01:315-343: the
+//                 reference collides the source cell once per destination; the moments
+//                 and the update do not depend on which destination asks)
+//   k_a_stream    interior fluid cells: pull the post-collision value, or bounce the cell's
+//                 own pre-collision value back when the source is solid, blend the sponge
+//                 of the destination, rho = sum in slot order (01:308-356)
+// Populations are SoA in ordering A (01:17-19), f[q*ncell + cell], cell as in GridDesc.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace nlos {
+
+namespace {
+
+constexpr int EXA[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int EYA[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int EZA[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int INVA[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+// get_feq (01:86-91), including the products with zero velocity components.
+__device__ __forceinline__ double feq_a(int k, double r, double ux, double uy, double uz) {
+    const double cu = (double)EXA[k] * ux + (double)EYA[k] * uy + (double)EZA[k] * uz;
+    const double u2 = ux * ux + uy * uy + uz * uz;
+    return W64[k] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+}
+
+__device__ __forceinline__ long long off_a(const GridDesc &g, int q) {
+    return (long long)EXA[q] + (long long)EYA[q] * g.px + (long long)EZA[q] * g.plane;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_a_halo_xy(double *f, GridDesc g) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long ring = 2LL * (g.nx + 2) + 2LL * g.ny;   // cells with i or j on the halo
+    if (t >= ring * g.nz) return;
+    const int k = (int)(t / ring) + 1;
+    long long r = t % ring;
+    int i, j;
+    if (r < g.nx + 2) { i = (int)r; j = 0; }
+    else if ((r -= g.nx + 2) < g.nx + 2) { i = (int)r; j = g.ny + 1; }
+    else if ((r -= g.nx + 2) < g.ny) { i = 0; j = (int)r + 1; }
+    else { i = g.nx + 1; j = (int)(r - g.ny) + 1; }
+    const long long c = cell_index(g, i, j, k);
+    // feq(q, 1, 0, 0, 0) = w_q exactly (every term of the bracket but 1 is zero)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[(long long)q * g.ncell + c] = W64[q];
+}
+
+__global__ __launch_bounds__(256) void k_a_halo_z(double *f, GridDesc g) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long plane = (long long)(g.nx + 2) * (g.ny + 2);
+    if (t >= plane) return;
+    const int i = (int)(t % (g.nx + 2)), j = (int)(t / (g.nx + 2));
+    const long long lo = cell_index(g, i, j, 0), hi = cell_index(g, i, j, g.nz + 1);
+    const long long top = cell_index(g, i, j, g.nz), bot = cell_index(g, i, j, 1);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const long long o = (long long)q * g.ncell;
+        f[o + lo] = f[o + top];
+        f[o + hi] = f[o + bot];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_a_collide(const double *f, double *post, const uint8_t *mask, GridDesc g,
+                                                   double tau) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    if (t >= total) return;
+    const int i = (int)(t % (g.nx + 2));
+    const int j = (int)((t / (g.nx + 2)) % (g.ny + 2));
+    const int k = (int)(t / ((long long)(g.nx + 2) * (g.ny + 2)));
+    const long long c = cell_index(g, i, j, k);
+    if (mask[c] & MK_SOLID) return;
+    double v[Q];
+#pragma unroll
+    for (int p = 0; p < Q; ++p) v[p] = f[(long long)p * g.ncell + c];
+    double sr = 0.0, sux = 0.0, suy = 0.0, suz = 0.0;
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+        sr += v[p];
+        sux += v[p] * (double)EXA[p];
+        suy += v[p] * (double)EYA[p];
+        suz += v[p] * (double)EZA[p];
+    }
+    if (sr == 0.0) sr = 1.0;
+    sux /= sr;
+    suy /= sr;
+    suz /= sr;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double fe = feq_a(q, sr, sux, suy, suz);
+        post[(long long)q * g.ncell + c] = v[q] - (v[q] - fe) / tau;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_a_stream(AStepArgs a) {
+    const GridDesc &g = a.g;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    if (n >= total) return;
+    const int i = (int)(n % g.nx) + 1;
+    const int j = (int)((n / g.nx) % g.ny) + 1;
+    const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
+    const long long c = cell_index(g, i, j, k);
+    const uint8_t m = a.mask[c];
+    if (m & MK_SOLID) {
+        if (a.field) a.field[n] = 1.0;   // 01:367 writes rho = 1 for solid cells
+        return;
+    }
+    const double s = a.sigma[n];
+    const bool in_sponge = s > 1e-6;
+    double out[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const long long src = c - off_a(g, q);
+        if (a.mask[src] & MK_SOLID) {
+            out[q] = a.f[(long long)INVA[q] * g.ncell + c];
+        } else {
+            const double p = a.post[(long long)q * g.ncell + src];
+            out[q] = in_sponge ? (1.0 - s) * p + s * W64[q] : p;   // feq(q, 1, 0, 0, 0) = w_q
+        }
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        a.f[(long long)q * g.ncell + c] = out[q];
+        sum += out[q];
+    }
+    if (!isfinite(sum)) atomicOr(a.flag, 1);
+    if (a.field) a.field[n] = sum;
+    if ((m & MK_PROBE) && a.probe_row) {
+        int lo = 0, hi = a.n_probe_cells - 1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const long long v = a.probe_cells[mid];
+            if (v == c) { a.probe_row[mid] = sum - 1.0; break; }
+            if (v < c) lo = mid + 1; else hi = mid - 1;
+        }
+    }
+}
+
+hipError_t launch_a_halo(double *f, const GridDesc &g, hipStream_t s) {
+    const long long ring = (2LL * (g.nx + 2) + 2LL * g.ny) * g.nz;
+    k_a_halo_xy<<<(unsigned)((ring + 255) / 256), 256, 0, s>>>(f, g);
+    const long long plane = (long long)(g.nx + 2) * (g.ny + 2);
+    k_a_halo_z<<<(unsigned)((plane + 255) / 256), 256, 0, s>>>(f, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_a_sweep(const AStepArgs &a, hipStream_t s) {
+    const GridDesc &g = a.g;
+    const long long all = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    k_a_collide<<<(unsigned)((all + 255) / 256), 256, 0, s>>>(a.f, a.post, a.mask, g, a.tau);
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    k_a_stream<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace nlos

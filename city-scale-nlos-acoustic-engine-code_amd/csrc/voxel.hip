@@ -190,6 +190,22 @@ hipError_t launch_mask_from_votes(const VoxelArgs &v, const GridDesc &g, const u
     return hipGetLastError();
 }
 
+// Family A (z periodic): the z-halo planes take the SOLID flags of planes nz and 1, as 01
+// evaluates its geometry on the halo cells too (01:170-180).
+__global__ __launch_bounds__(256) void k_mask_periodic_z(uint8_t *mask, GridDesc g) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)g.nx * g.ny) return;
+    const int i = (int)(t % g.nx) + 1, j = (int)(t / g.nx) + 1;
+    mask[cell_index(g, i, j, 0)] = mask[cell_index(g, i, j, g.nz)] & MK_SOLID;
+    mask[cell_index(g, i, j, g.nz + 1)] = mask[cell_index(g, i, j, 1)] & MK_SOLID;
+}
+
+hipError_t launch_mask_periodic_z(uint8_t *mask, const GridDesc &g, hipStream_t s) {
+    const long long n = (long long)g.nx * g.ny;
+    k_mask_periodic_z<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(mask, g);
+    return hipGetLastError();
+}
+
 // NEAR bit on fluid interior cells with a solid neighbour among the 18 links; keeps the
 // SOLID and PROBE bits.
 __global__ __launch_bounds__(256) void k_mask_flags(uint8_t *mask, GridDesc g) {

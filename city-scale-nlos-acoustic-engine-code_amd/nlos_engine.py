@@ -22,6 +22,7 @@ LIB_PATH = os.environ.get("NLOS_ENGINE_LIB") or os.path.join(HERE, "libnlos_engi
 F64, F32, F16S, B16 = 0, 1, 2, 3
 VTK_ASCII, VTK_BINARY = 0, 1
 VOXEL_HEIGHT, VOXEL_PARITY, VOXEL_WINDING, VOXEL_3AXIS = 0, 1, 2, 4
+FAMILY_B, FAMILY_A = 0, 1
 PRECISIONS = {"f64": F64, "f32": F32, "f16s": F16S, "b16": B16}
 FACE_XM, FACE_XP, FACE_YM, FACE_YP, FACE_ZM, FACE_ZP = 1, 2, 4, 8, 16, 32
 FACES_REFERENCE = FACE_XM | FACE_XP | FACE_YM | FACE_YP | FACE_ZP
@@ -46,7 +47,8 @@ class Config(C.Structure):
                 ("sponge_faces", C.c_int32), ("nscbc_faces", C.c_int32), ("ground", C.c_int32),
                 ("use_geometry", C.c_int32), ("output_world_coords", C.c_int32),
                 ("precision", C.c_int32), ("f16_scale_log2", C.c_int32),
-                ("stream_pattern", C.c_int32), ("voxelizer", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("stream_pattern", C.c_int32), ("voxelizer", C.c_int32), ("family", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class Source(C.Structure):
@@ -98,6 +100,7 @@ SIGNATURES = {
     "nlos_set_mask": (C.c_int, [_E, _P]),
     "nlos_get_mask": (C.c_int, [_E, _P]),
     "nlos_set_density": (C.c_int, [_E, _P]),
+    "nlos_set_sponge": (C.c_int, [_E, _P]),
     "nlos_set_source": (C.c_int, [_E, C.POINTER(Source)]),
     "nlos_set_probes": (C.c_int, [_E, _P, C.c_int32, _P]),
     "nlos_apply_source": (C.c_int, [_E, C.c_double]),
@@ -274,6 +277,12 @@ class Engine:
         r = np.ascontiguousarray(rho, dtype=np.float64)
         assert r.shape == tuple(reversed(self.dims))
         _check(self.lib.nlos_set_density(self.h, _ptr(r)))
+
+    def set_sponge(self, sigma: np.ndarray) -> None:
+        """FAMILY_A only: sponge strength per interior cell."""
+        s = np.ascontiguousarray(sigma, dtype=np.float64)
+        assert s.shape == tuple(reversed(self.dims))
+        _check(self.lib.nlos_set_sponge(self.h, _ptr(s)))
 
     def set_source(self, src: Source, samples: Optional[np.ndarray] = None) -> None:
         if samples is not None:

@@ -86,8 +86,30 @@ typedef struct nlos_config {
     int32_t f16_scale_log2;    /* NLOS_F16S: stored value = g * 2^k; 0 selects the default 24 */
     int32_t stream_pattern;    /* nlos_pattern */
     int32_t voxelizer;         /* nlos_voxelizer; 0 is the reference's height field */
-    int32_t reserved[3];
+    int32_t family;            /* nlos_family; 0 is the 05/08 solver */
+    int32_t reserved[2];
 } nlos_config;
+
+/* Which of the reference's two solvers the engine runs.
+ * NLOS_FAMILY_B: 05_sensor_illusion / 08_city_scale_uav (everything above).
+ * NLOS_FAMILY_A: the legacy step_pull of 01_acoustic_scattering.cpp and
+ *   04_hpc_scalability.cpp, bit for bit in FP64:
+ *   - velocity ordering A (01:17-19);
+ *   - the source cell collides (BGK, f - (f - feq)/tau);
+ *   - a solid source returns the destination's own pre-collision f[inv q];
+ *   - the destination's sponge blend follows;
+ *   - rho is the sum of the new populations (01:303-359);
+ *   - x and y halos at rest, z periodic (01:110-112, 230-301);
+ *   - probes at cell (int)((x - min)/dx - 0.5) record rho - 1 (01:448-479);
+ *   - dt = sqrt(1/3) dx / c (01:37-41), nz at least 3 (01:100).
+ *   The sponge is per cell: by default 01's rule on the x/y faces, with thickness
+ *   sponge_width and strength sponge_strength (01:207-225); nlos_set_sponge replaces it
+ *   (04's radial ramp).
+ *   Needs NLOS_F64 and the two-array pattern.  Sources, NSCBC, the ground halo, batches
+ *   and the adjoint are family-B features and are refused or ignored (ground and NSCBC
+ *   are off).  nlos_write_vtk writes 01's format (01:414-436, scale 100);
+ *   nlos_write_probe_dat writes 01's files including the t = 0 line. */
+typedef enum nlos_family { NLOS_FAMILY_B = 0, NLOS_FAMILY_A = 1 } nlos_family;
 
 /* How an STL mesh becomes solid cells.
  * NLOS_VOXEL_HEIGHT is the reference: the top hit of a vertical ray per column, every cell
@@ -192,6 +214,9 @@ int nlos_set_mask(nlos_engine *e, const uint8_t *mask);
 int nlos_get_mask(nlos_engine *e, uint8_t *mask);
 
 /* ---- initial state, source, probes --------------------------------------------------- */
+/* NLOS_FAMILY_A only: sponge strength per interior cell, x fastest (replaces 01's default
+ * rule; 04:176-208 is a radial one).  A cell blends when sigma > 1e-6 (01:313). */
+int nlos_set_sponge(nlos_engine *e, const double *sigma);
 /* Initial populations f = feq(rho, 0) = w_q*rho on fluid cells (interior rho, x fastest). */
 int nlos_set_density(nlos_engine *e, const double *rho);
 int nlos_set_source(nlos_engine *e, const nlos_source *src);

@@ -1,4 +1,4 @@
-"""Writes golden fixtures by running the compiled, unmodified reference program.
+"""Writes golden fixtures by running the compiled, unmodified reference programs.
 
 Runs `oracle/_ref/05_sensor_illusion` (built by `make -C oracle ref` from
 /root/reference/05_sensor_illusion.cpp) twice in a scratch directory:
@@ -84,14 +84,58 @@ def run_scene(name: str, stl: bytes | None, exe: str, threads: int) -> None:
         shutil.rmtree(work, ignore_errors=True)
 
 
+# 01_acoustic_scattering (family A): 320x320x8 cells, 402 probe records, frames at steps
+# 0, 104 and 208 (01:502-524).  Kept: probe_<A|B|C>.dat byte for byte, and of each frame the
+# header and every 8th cell in x and y of the z layer 4.
+REF01_GRID = (320, 320, 8)
+REF01_FRAMES = (0, 104, 208)
+REF01_LAYER, REF01_STRIDE = 4, 8
+
+
+def sample_vtk01(path: str):
+    nx, ny, _ = REF01_GRID
+    with open(path) as fh:
+        header = [next(fh).rstrip("\n") for _ in range(10)]
+        want = {(REF01_LAYER * ny + j) * nx + i for j in range(0, ny, REF01_STRIDE)
+                for i in range(0, nx, REF01_STRIDE)}
+        values = {}
+        for n, line in enumerate(fh):
+            if n in want:
+                values[n] = line.strip()
+    vals = [values[(REF01_LAYER * ny + j) * nx + i] for j in range(0, ny, REF01_STRIDE)
+            for i in range(0, nx, REF01_STRIDE)]
+    return {"header": header, "layer": REF01_LAYER, "stride": REF01_STRIDE, "values": vals}
+
+
+def run_ref01(exe: str, threads: int) -> None:
+    dest = os.path.join(HERE, "ref01")
+    os.makedirs(dest, exist_ok=True)
+    work = tempfile.mkdtemp(prefix="ref01_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        with open(os.path.join(dest, "stdout.txt"), "w") as log:
+            subprocess.run([exe], cwd=work, env=env, check=True, stdout=log, stderr=subprocess.STDOUT)
+        for p in ("A", "B", "C"):
+            shutil.copy(os.path.join(work, "output_mpi", f"probe_{p}.dat"), os.path.join(dest, f"probe_{p}.dat"))
+        frames = {str(f): sample_vtk01(os.path.join(work, "output_mpi", f"cyl_{f:05d}.vtk")) for f in REF01_FRAMES}
+        with open(os.path.join(dest, "vtk_frames.json"), "w") as fh:
+            json.dump({"grid": list(REF01_GRID), "frames": frames}, fh, indent=0)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main() -> None:
-    exe = os.path.join(REPO, "oracle", "_ref", "05_sensor_illusion")
-    if not os.path.exists(exe):
-        subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    scenes = sys.argv[1:] or ["flat", "city"]
+    scenes = sys.argv[1:] or ["flat", "city", "ref01"]
     for s in scenes:
-        run_scene(s, None if s == "flat" else city_stl.reference_city_stl(), exe, threads)
+        prog = "01_acoustic_scattering" if s == "ref01" else "05_sensor_illusion"
+        exe = os.path.join(REPO, "oracle", "_ref", prog)
+        if not os.path.exists(exe):
+            subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
+        if s == "ref01":
+            run_ref01(exe, threads)
+        else:
+            run_scene(s, None if s == "flat" else city_stl.reference_city_stl(), exe, threads)
         print("done", s, flush=True)
 
 

@@ -179,3 +179,101 @@ class Oracle:
     def write_f(self, f):
         a = np.ascontiguousarray(f, dtype=np.float64)
         self.L.oracle_write_f(self.h, _p(a))
+
+
+_lib_a = None
+
+
+def lib_a() -> C.CDLL:
+    """The family-A restatement (oracle/ref_cpu_a.cpp) in the same library."""
+    global _lib_a
+    if _lib_a is None:
+        L = lib()
+        vp = C.c_void_p
+        sig = {
+            "oracle_a_create": (vp, [C.POINTER(nlos.Config)]),
+            "oracle_a_destroy": (None, [vp]),
+            "oracle_a_dims": (None, [vp, vp]),
+            "oracle_a_set_mask": (None, [vp, vp]),
+            "oracle_a_set_sponge": (None, [vp, vp]),
+            "oracle_a_get_sponge": (None, [vp, vp]),
+            "oracle_a_set_density": (None, [vp, vp]),
+            "oracle_a_set_probes": (C.c_int, [vp, vp, C.c_int32]),
+            "oracle_a_run": (None, [vp, C.c_longlong]),
+            "oracle_a_read_probes": (C.c_longlong, [vp, vp]),
+            "oracle_a_read_f": (None, [vp, vp]),
+            "oracle_a_read_rho": (None, [vp, vp]),
+        }
+        for n, (r, a) in sig.items():
+            fn = getattr(L, n)
+            fn.restype = r
+            fn.argtypes = a
+        _lib_a = L
+    return _lib_a
+
+
+class OracleA:
+    """FP64 CPU restatement of the family-A solver of 01/04 (oracle/ref_cpu_a.cpp)."""
+
+    def __init__(self, cfg):
+        self.L = lib_a()
+        self.h = self.L.oracle_a_create(C.byref(cfg))
+        d = np.zeros(3, np.int32)
+        self.L.oracle_a_dims(self.h, _p(d))
+        self.dims = tuple(int(v) for v in d)
+        self.n_probes = 0
+
+    def close(self):
+        if self.h:
+            self.L.oracle_a_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mask(self, solid):
+        m = np.ascontiguousarray(solid, dtype=np.uint8)
+        self.L.oracle_a_set_mask(self.h, _p(m))
+
+    def set_sponge(self, sigma):
+        s = np.ascontiguousarray(sigma, dtype=np.float64)
+        self.L.oracle_a_set_sponge(self.h, _p(s))
+
+    def sponge(self):
+        nx, ny, nz = self.dims
+        out = np.empty((nz, ny, nx), np.float64)
+        self.L.oracle_a_get_sponge(self.h, _p(out))
+        return out
+
+    def set_density(self, rho):
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        self.L.oracle_a_set_density(self.h, _p(r))
+
+    def set_probes(self, xyz):
+        a = np.ascontiguousarray(np.asarray(xyz, np.float64).reshape(-1, 3))
+        self.n_probes = len(a)
+        return self.L.oracle_a_set_probes(self.h, _p(a), len(a))
+
+    def run(self, n):
+        self.L.oracle_a_run(self.h, n)
+
+    def read_probes(self):
+        """rho - 1 per probe at s = 0 (initial) .. steps."""
+        buf = np.zeros(self.n_probes * 100000, np.float64)
+        m = self.L.oracle_a_read_probes(self.h, _p(buf))
+        return buf[: self.n_probes * m].reshape(self.n_probes, m)
+
+    def read_f(self):
+        nx, ny, nz = self.dims
+        out = np.empty((nz, ny, nx, 19), np.float64)
+        self.L.oracle_a_read_f(self.h, _p(out))
+        return out
+
+    def read_rho(self):
+        nx, ny, nz = self.dims
+        out = np.empty((nz, ny, nx), np.float64)
+        self.L.oracle_a_read_rho(self.h, _p(out))
+        return out

@@ -1,0 +1,154 @@
+"""Legacy family-A mode (NLOS_FAMILY_A: the step_pull of 01_acoustic_scattering.cpp and
+04_hpc_scalability.cpp) on the GPU.
+
+* 01's own run reproduces the compiled reference's probe_<A|B|C>.dat files byte for byte
+  (all 402 records) and its VTK frames at steps 104 and 208 (sampled values, header);
+* on smaller scenes the engine's populations equal the oracle's (oracle/ref_cpu_a.cpp)
+  bit for bit, with the default and with a radial (04-style) sponge;
+* checkpoint/restore is bitwise; family-B-only features are refused.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import family_a as fa
+import nlos_engine as nlos
+from oracle_ffi import OracleA
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref01")
+
+
+def _engine(cfg, solid, rho, names=None, pts=None):
+    e = nlos.Engine(cfg)
+    e.set_mask(solid)
+    if rho is not None:
+        e.set_density(rho)
+    if pts is not None:
+        e.set_probes(pts, names)
+    return e
+
+
+def test_ref01_probe_files_are_byte_identical(gpu, tmp_path):
+    cfg = fa.config()
+    solid, rho = fa.cylinder_and_pulse(cfg)
+    names, pts = fa.ref01_probes()
+    e = _engine(cfg, solid, rho, names, pts)
+    n = fa.ref01_steps(cfg)                    # records s = 0 .. n
+    print(f"[family A] 01 run: {n} steps", flush=True)
+    e.step(n)
+    e.synchronize()
+    e.write_probe_dat(str(tmp_path), scale=100.0)
+    for name in names:
+        got = (tmp_path / f"probe_{name}.dat").read_text()
+        want = open(os.path.join(GOLDEN, f"probe_{name}.dat")).read()
+        assert got == want, name
+
+
+def test_ref01_vtk_frames(gpu, tmp_path):
+    frames = json.load(open(os.path.join(GOLDEN, "vtk_frames.json")))
+    nx, ny, nz = frames["grid"]
+    cfg = fa.config()
+    solid, rho = fa.cylinder_and_pulse(cfg)
+    e = _engine(cfg, solid, rho)
+    done = 0
+    for f in ("104", "208"):
+        want = frames["frames"][f]
+        e.step(int(f) - done, capture_field=True)
+        done = int(f)
+        path = tmp_path / f"cyl_{f}.vtk"
+        e.write_vtk(str(path))
+        lines = path.read_text().split("\n")
+        assert lines[:10] == want["header"], f
+        k, st = want["layer"], want["stride"]
+        got = [lines[10 + (k * ny + j) * nx + i] for j in range(0, ny, st) for i in range(0, nx, st)]
+        assert got == want["values"], f
+
+
+def _small_scene(seed=4):
+    cfg = fa.config(x=(-1.2, 1.2), y=(-1.0, 1.0), z=(-0.15, 0.15), dx=0.05, tau=0.52, sponge=0.3, strength=0.8)
+    nx, ny, nz = fa.dims(cfg)
+    solid, rho = fa.cylinder_and_pulse(cfg, radius=0.25, centre=(0.5, 0.1), width=0.3, amp=0.05)
+    rng = np.random.default_rng(seed)
+    solid[:, 3:7, 30:36] = 1                     # a block touching nothing, z-invariant
+    solid[1, 15:18, 8:12] = 1                    # a block in one z plane (periodic neighbours)
+    rho = rho + 1e-4 * rng.standard_normal(rho.shape) * (solid == 0)
+    pts = [(0.0, 0.5, 0.0), (-0.7, -0.4, 0.1), (0.9, 0.8, -0.1)]
+    return cfg, solid, rho, pts
+
+
+@pytest.mark.parametrize("radial", [False, True])
+def test_engine_equals_oracle_bitwise(gpu, radial):
+    cfg, solid, rho, pts = _small_scene()
+    e = _engine(cfg, solid, rho, None, pts)
+    o = OracleA(cfg)
+    o.set_mask(solid)
+    o.set_density(rho)
+    o.set_probes(pts)
+    if radial:
+        # 04:182-206: sigma = s * min(1, (r - R0)/T)^2 beyond R0
+        nx, ny, nz = fa.dims(cfg)
+        sg = np.zeros((nz, ny, nx))
+        for j in range(ny):
+            for i in range(nx):
+                x = cfg.x_min + (i + 0.5) * cfg.dx
+                y = cfg.y_min + (j + 0.5) * cfg.dx
+                r = math.sqrt(x * x + y * y)
+                if r > 0.6:
+                    ratio = min(1.0, (r - 0.6) / 0.3)
+                    sg[:, j, i] = 0.5 * ratio * ratio
+        sg[solid == 1] = 0.0
+        e.set_sponge(sg)
+        o.set_sponge(sg)
+    n = 57
+    e.step(n)
+    o.run(n)
+    e.synchronize()
+    fe, fo = e.debug_read_f(), o.read_f()
+    fluid = solid == 0
+    assert np.array_equal(fe[fluid], fo[fluid])
+    po = o.read_probes()
+    assert np.array_equal(e.read_probes(), po[:, 1:])
+    assert np.abs(po).max() > 0
+
+
+def test_checkpoint_restore_is_bitwise(gpu):
+    cfg, solid, rho, pts = _small_scene()
+    e = _engine(cfg, solid, rho, None, pts)
+    e.step(11)
+    blob = e.checkpoint()
+    e.step(13)
+    e.synchronize()
+    p1, f1 = e.read_probes(), e.debug_read_f()
+    r = _engine(cfg, solid, None, None, pts)
+    r.restore(blob)
+    r.step(13)
+    r.synchronize()
+    assert np.array_equal(r.read_probes(), p1)
+    assert np.array_equal(r.debug_read_f(), f1, equal_nan=True)
+
+
+def test_family_b_features_are_refused(gpu):
+    cfg = fa.config(x=(-0.5, 0.5), y=(-0.5, 0.5), z=(-0.1, 0.1), dx=0.05)
+    cfg32 = fa.config(x=(-0.5, 0.5), y=(-0.5, 0.5), z=(-0.1, 0.1), dx=0.05)
+    cfg32.precision = nlos.F32
+    with pytest.raises(nlos.EngineError) as ex:
+        nlos.Engine(cfg32)
+    assert ex.value.code == 7
+    e = nlos.Engine(cfg)
+    with pytest.raises(nlos.EngineError) as ex:
+        e.set_source(nlos.default_source())
+    assert ex.value.code == 7
+    with pytest.raises(nlos.EngineError) as ex:
+        nlos.Batch([cfg])
+    assert ex.value.code == 7
+    b = nlos.Engine(nlos.default_config(x_max=10.0, y_max=10.0, z_max=10.0, dx=1.0))
+    with pytest.raises(nlos.EngineError) as ex:
+        b.set_sponge(np.zeros((10, 10, 10)))
+    assert ex.value.code == 7
