@@ -668,10 +668,14 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
     }
     const auto nxt = glb(static_cast<S *>(a.pop_out));
     if constexpr (MODE == M_B16) {
-        (glb(b16_rest(a.pop_out)) + cb)[t] = f[0];
+        // Non-temporal stores: the written lines are not read again this step.  Measured
+        // 4.95 -> 4.73 ms at 1024x1024x256 with the L2 fetch volume unchanged, so the gain
+        // is on the write path; the same hint on the loads, or on the other storage modes'
+        // stores, was slower
+        __builtin_nontemporal_store(f[0], glb(b16_rest(a.pop_out)) + cb + t);
         const auto nm = glb(b16_mants(a.pop_out, N));
 #pragma unroll
-        for (int q = 1; q < Q; ++q) (nm + ((long long)q * N + cb))[t] = bm[q];
+        for (int q = 1; q < Q; ++q) __builtin_nontemporal_store(bm[q], nm + ((long long)q * N + cb) + t);
         (glb(b16_exps(a.pop_out, N)) + cb)[t] = (signed char)(be - B16_BITS);
     } else {
 #pragma unroll
