@@ -62,6 +62,14 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// A DevBuf local to one call, freed on every return path.
+struct ScopedBuf : DevBuf {
+    ScopedBuf() = default;
+    ScopedBuf(const ScopedBuf &) = delete;
+    ScopedBuf &operator=(const ScopedBuf &) = delete;
+    ~ScopedBuf() { release(); }
+};
+
 struct Triangle13 { double v[13]; };   // v1, v2, v3, min x, min y, max x, max y
 
 // Binary STL parse and 200x200 binning (GeometryEngine::load/buildGrid, 05:70-142).
@@ -1075,7 +1083,7 @@ int voxelize_geometry(nlos_engine *e, HostGeometry &geo, bool ok) {
         HIP_TRY(launch_mask_from_heights(v, e->g, e->gh.as<double>(), e->mask.as<uint8_t>(), e->stream));
     } else {
         const size_t n = (size_t)e->n_interior();
-        DevBuf votes, flag, bstart, blist;
+        ScopedBuf votes, flag, bstart, blist;
         HIP_TRY(votes.reserve(n));
         HIP_TRY(flag.reserve(4));
         HIP_TRY(launch_fill_u8(votes.as<uint8_t>(), (long long)n, 0, e->stream));
@@ -1115,7 +1123,6 @@ int voxelize_geometry(nlos_engine *e, HostGeometry &geo, bool ok) {
         int over = 0;
         HIP_TRY(hipStreamSynchronize(e->stream));
         HIP_TRY(hipMemcpy(&over, flag.p, 4, hipMemcpyDeviceToHost));
-        votes.release(); flag.release(); bstart.release(); blist.release();
         if (over) return fail(NLOS_ERR_UNSUPPORTED, "a grid line crosses the mesh more than 256 times");
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1568,7 +1575,7 @@ NLOS_API int nlos_write_probe_dat(nlos_engine *e, const char *dir, const char *c
 namespace {
 struct CkptHeader {
     char magic[8];
-    int32_t version, mode;
+    int32_t version, mode, family, reserved;
     int32_t nx, ny, nz, lay;
     int64_t step, ncell, n_solid, n_ucells;
     uint64_t pop_bytes, state_bytes, hist_bytes;
@@ -1579,7 +1586,7 @@ NLOS_API int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *need
     if (int rc = ready(e)) return rc;
     CkptHeader h{};
     std::memcpy(h.magic, "NLOSCKP1", 8);
-    h.version = 1; h.mode = e->mode;
+    h.version = 2; h.mode = e->mode; h.family = e->family;
     h.nx = e->g.nx; h.ny = e->g.ny; h.nz = e->g.nz; h.lay = e->lay;
     h.step = e->step; h.ncell = e->g.ncell; h.n_solid = e->n_solid;
     h.n_ucells = (int64_t)e->probe_ucells.size();
@@ -1609,6 +1616,8 @@ NLOS_API int nlos_restore(nlos_engine *e, const void *buf, size_t len) {
     CkptHeader h;
     std::memcpy(&h, buf, sizeof h);
     if (std::memcmp(h.magic, "NLOSCKP1", 8) != 0) return fail(NLOS_ERR_ARG, "not a checkpoint");
+    if (h.version != 2) return fail(NLOS_ERR_ARG, "checkpoint format version %d is not 2", h.version);
+    if (h.family != e->family) return fail(NLOS_ERR_ARG, "checkpoint was taken with the other solver family");
     if ((h.lay == LAY_AB) != (e->pattern == NLOS_PATTERN_TWO_ARRAY))
         return fail(NLOS_ERR_ARG, "checkpoint was taken with the other stream pattern");
     if (h.mode != e->mode || h.nx != e->g.nx || h.ny != e->g.ny || h.nz != e->g.nz || h.ncell != e->g.ncell ||
@@ -1663,14 +1672,13 @@ NLOS_API int nlos_timing_get(nlos_engine *e, nlos_timing *t) {
 // [cell][q], NaN on solid cells.
 NLOS_API int nlos_debug_read_f(nlos_engine *e, double *out) {
     if (int rc = ready(e)) return rc;
-    DevBuf tmp;
+    ScopedBuf tmp;
     HIP_TRY(tmp.reserve((size_t)e->n_interior() * Q * 8));
     HIP_TRY(by_mode(e->mode, [&](auto L) {
         return decltype(L)::gather_f(e->curp(), e->mask.as<uint8_t>(), e->g, e->lay, e->inv_scale, tmp.as<double>(), e->stream);
     }));
     HIP_TRY(hipStreamSynchronize(e->stream));
     HIP_TRY(hipMemcpy(out, tmp.p, (size_t)e->n_interior() * Q * 8, hipMemcpyDeviceToHost));
-    tmp.release();
     return NLOS_OK;
 }
 
@@ -1735,7 +1743,7 @@ NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, in
 NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) {
     if (bytes < 16 || reps < 1 || !avg_ms) return fail(NLOS_ERR_ARG, "bad copy request");
     const long long n16 = (long long)(bytes / 16);
-    DevBuf a, b;
+    ScopedBuf a, b;
     HIP_TRY(a.reserve((size_t)n16 * 16));
     HIP_TRY(b.reserve((size_t)n16 * 16));
     hipStream_t s;
@@ -1755,8 +1763,6 @@ NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
-    a.release();
-    b.release();
     return NLOS_OK;
 }
 
@@ -2213,8 +2219,8 @@ NLOS_API int nlos_record_forward(nlos_engine *e, int64_t n, int32_t every) {
             e->rec_start = -1;
             return fail(NLOS_ERR_NOMEM, "cannot allocate checkpoint %zu", e->ck.size());
         }
+        e->ck.push_back(b);   // owned by the engine from here on, whatever fails next
         HIP_TRY(hipMemcpyAsync(b.p, e->curp(), pop_bytes(e), hipMemcpyDeviceToDevice, e->stream));
-        e->ck.push_back(b);
         if (int rc = nlos_step(e, std::min<long long>(every, n - done), 0)) return rc;
         e->rec_steps += std::min<long long>(every, n - done);
     }

@@ -152,3 +152,20 @@ def test_family_b_features_are_refused(gpu):
     with pytest.raises(nlos.EngineError) as ex:
         b.set_sponge(np.zeros((10, 10, 10)))
     assert ex.value.code == 7
+
+
+def test_checkpoint_of_the_other_family_is_refused(gpu):
+    """Both families store FP64 populations, but in different velocity orderings."""
+    cfg, solid, rho, pts = _small_scene()
+    a = _engine(cfg, solid, rho, None, pts)
+    a.step(2)
+    blob = a.checkpoint()
+    cfg_b = fa.config(x=(-1.2, 1.2), y=(-1.0, 1.0), z=(-0.15, 0.15), dx=0.05)
+    cfg_b.family = nlos.FAMILY_B
+    cfg_b.stream_pattern = nlos.PATTERN_TWO_ARRAY
+    cfg_b.ground = 0
+    cfg_b.nscbc_faces = 0
+    b = _engine(cfg_b, solid, None, None, pts)
+    with pytest.raises(nlos.EngineError) as ex:
+        b.restore(blob)
+    assert ex.value.code == 2
