@@ -1572,7 +1572,10 @@ NLOS_API int nlos_write_probe_dat(nlos_engine *e, const char *dir, const char *c
 }
 
 // ---- checkpoint / restore -------------------------------------------------------------------
+// Format version 3: populations blocked by 64 cells (pop_index); version 2 held them slot by
+// slot and is refused.
 namespace {
+constexpr int32_t CKPT_VERSION = 3;
 struct CkptHeader {
     char magic[8];
     int32_t version, mode, family, reserved;
@@ -1586,7 +1589,7 @@ NLOS_API int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *need
     if (int rc = ready(e)) return rc;
     CkptHeader h{};
     std::memcpy(h.magic, "NLOSCKP1", 8);
-    h.version = 2; h.mode = e->mode; h.family = e->family;
+    h.version = CKPT_VERSION; h.mode = e->mode; h.family = e->family;
     h.nx = e->g.nx; h.ny = e->g.ny; h.nz = e->g.nz; h.lay = e->lay;
     h.step = e->step; h.ncell = e->g.ncell; h.n_solid = e->n_solid;
     h.n_ucells = (int64_t)e->probe_ucells.size();
@@ -1616,7 +1619,8 @@ NLOS_API int nlos_restore(nlos_engine *e, const void *buf, size_t len) {
     CkptHeader h;
     std::memcpy(&h, buf, sizeof h);
     if (std::memcmp(h.magic, "NLOSCKP1", 8) != 0) return fail(NLOS_ERR_ARG, "not a checkpoint");
-    if (h.version != 2) return fail(NLOS_ERR_ARG, "checkpoint format version %d is not 2", h.version);
+    if (h.version != CKPT_VERSION)
+        return fail(NLOS_ERR_ARG, "checkpoint format version %d is not %d", h.version, CKPT_VERSION);
     if (h.family != e->family) return fail(NLOS_ERR_ARG, "checkpoint was taken with the other solver family");
     if ((h.lay == LAY_AB) != (e->pattern == NLOS_PATTERN_TWO_ARRAY))
         return fail(NLOS_ERR_ARG, "checkpoint was taken with the other stream pattern");
@@ -1689,7 +1693,7 @@ NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, in
     if (i < 0 || i >= e->g.nx || j < 0 || j >= e->g.ny || k < 0 || k >= e->g.nz || q < 0 || q >= Q)
         return fail(NLOS_ERR_ARG, "poke outside grid");
     HIP_TRY(hipStreamSynchronize(e->stream));
-    const long long idx = (long long)q * e->g.ncell + e->cell(i + 1, j + 1, k + 1);
+    const long long idx = pop_index_host(e->mode, q, e->cell(i + 1, j + 1, k + 1), e->g.ncell);
     e->fb_valid = false;
     e->walls_valid = false;
     if (e->mode == M_F64) {
@@ -1702,12 +1706,12 @@ NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, in
         const long long N = e->g.ncell, c = e->cell(i + 1, j + 1, k + 1);
         char *base = static_cast<char *>(e->curp());
         float *rp = reinterpret_cast<float *>(base) + c;
-        short *mp = reinterpret_cast<short *>(base) + N;
+        short *mp = reinterpret_cast<short *>(base) + 2 * N;
         signed char *ep = reinterpret_cast<signed char *>(base) + 40 * N + c;
         short m[Q] = {0};
         signed char E;
         float x0;
-        for (int r = 1; r < Q; ++r) HIP_TRY(hipMemcpy(&m[r], mp + (size_t)r * N + c, 2, hipMemcpyDeviceToHost));
+        for (int r = 1; r < Q; ++r) HIP_TRY(hipMemcpy(&m[r], mp + mant_index(r, c), 2, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&E, ep, 1, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&x0, rp, 4, hipMemcpyDeviceToHost));
         double x[Q], mx = 0.0;
@@ -1730,7 +1734,7 @@ NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, in
             m[q] = 32767;
             E = 127;
         }
-        for (int r = 1; r < Q; ++r) HIP_TRY(hipMemcpy(mp + (size_t)r * N + c, &m[r], 2, hipMemcpyHostToDevice));
+        for (int r = 1; r < Q; ++r) HIP_TRY(hipMemcpy(mp + mant_index(r, c), &m[r], 2, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(ep, &E, 1, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(rp, &x0, 4, hipMemcpyHostToDevice));
     } else {

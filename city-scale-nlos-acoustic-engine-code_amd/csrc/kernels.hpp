@@ -57,14 +57,14 @@ __device__ __forceinline__ C wq(int q) { return static_cast<C>(W64[q]); }
 // max_{q>=1} |x_q| < 2^e; the mantissas are x_q * 2^(15 - e) rounded to nearest even and
 // saturated to 16 bits, and the stored byte is E = e - 15, so that x_q = m_q * 2^E.
 // Array of N = ncell cells: bytes [0, 4N) x_0 (FP32), [4N, 40N) mantissas (slot q >= 1 at
-// 16-bit index N + q*N + c), [40N, 41N) exponents.
+// 16-bit index 2N + mant_index(q, c)), [40N, 41N) exponents.
 
 __device__ __forceinline__ const float *b16_rest(const void *pop) { return static_cast<const float *>(pop); }
 __device__ __forceinline__ float *b16_rest(void *pop) { return static_cast<float *>(pop); }
 __device__ __forceinline__ const short *b16_mants(const void *pop, long long N) {
-    return static_cast<const short *>(pop) + N;
+    return static_cast<const short *>(pop) + 2 * N;
 }
-__device__ __forceinline__ short *b16_mants(void *pop, long long N) { return static_cast<short *>(pop) + N; }
+__device__ __forceinline__ short *b16_mants(void *pop, long long N) { return static_cast<short *>(pop) + 2 * N; }
 __device__ __forceinline__ const signed char *b16_exps(const void *pop, long long N) {
     return static_cast<const signed char *>(pop) + 40 * N;
 }
@@ -102,11 +102,11 @@ __device__ __forceinline__ void b16_load_g(const void *pop, long long N, long lo
     const int E = b16_exps(pop, N)[c];
     g[0] = b16_rest(pop)[c] * wq<float>(0);
 #pragma unroll
-    for (int q = 1; q < Q; ++q) g[q] = b16_dec(m[(long long)q * N + c], E) * wq<float>(q);
+    for (int q = 1; q < Q; ++q) g[q] = b16_dec(m[mant_index(q, c)], E) * wq<float>(q);
 }
 __device__ __forceinline__ float b16_slot_g(const void *pop, long long N, long long c, int q) {
     if (q == 0) return b16_rest(pop)[c] * wq<float>(0);
-    return b16_dec(b16_mants(pop, N)[(long long)q * N + c], b16_exps(pop, N)[c]) * wq<float>(q);
+    return b16_dec(b16_mants(pop, N)[mant_index(q, c)], b16_exps(pop, N)[c]) * wq<float>(q);
 }
 // Codes x into cell c; slots with write[q] false are left as they are (the ghost layer
 // writes only the slots that interior cells pull).  Returns e.
@@ -119,7 +119,7 @@ __device__ __forceinline__ int b16_store_x(void *pop, long long N, long long c, 
     short *mp = b16_mants(pop, N);
 #pragma unroll
     for (int q = 1; q < Q; ++q)
-        if ((write >> q) & 1u) mp[(long long)q * N + c] = m[q];
+        if ((write >> q) & 1u) mp[mant_index(q, c)] = m[q];
     b16_exps(pop, N)[c] = (signed char)(e - B16_BITS);
     return e;
 }
@@ -133,6 +133,45 @@ __device__ __forceinline__ void b16_g_to_x(const float *g, float *x) {
 __device__ __forceinline__ long long nb_off(const GridDesc &g, int q) {
     return (long long)EX[q] + (long long)EY[q] * g.px + (long long)EZ[q] * g.plane;
 }
+// Its y/z part, a multiple of 64 cells.
+__device__ __forceinline__ long long yz_off(const GridDesc &g, int q) {
+    return (long long)EY[q] * g.px + (long long)EZ[q] * g.plane;
+}
+
+// Slot addressing of a sweep lane.  A sweep tile's corner cell cb is a multiple of 64
+// (interior rows start at i = 1 with origin 63, tile widths are multiples of 64), so the
+// index of slot s of cell cb + u is b0 + s*ss + o(u) with b0 uniform over the workgroup and
+// o(u) >= 0 for u >= -64:
+//   blocked, NS slots per block: b0 = ((cb >> 6) - 1) * NS*64, ss = 64,
+//     o(u) = ((u + 64) >> 6) * NS*64 + (u & 63); a y/z offset d moves an index by d*NS;
+//   slot-major: b0 = cb - 64, ss = ncell, o(u) = u + 64; d moves an index by d.
+// o[1 + e] holds the lane's o(t - e) for e = -1, 0, 1: the cell itself and its two x
+// neighbours.
+template <int MODE, int NS = Q>
+struct Lanes {
+    static constexpr bool BLK = blocked_mode(MODE);
+    static constexpr long long YM = BLK ? NS : 1;
+    long long b0, ss;
+    unsigned o[3];
+    __device__ __forceinline__ Lanes(long long cb, unsigned t, long long ncell)
+        : b0(BLK ? ((cb >> 6) - 1) * (NS * PBLK) : cb - PBLK), ss(BLK ? PBLK : ncell) {
+#pragma unroll
+        for (int e = -1; e <= 1; ++e) {
+            const unsigned u = t - (unsigned)e + PBLK;   // t - e + 64 >= 63
+            o[1 + e] = BLK ? (u >> 6) * (NS * PBLK) + (u & (PBLK - 1)) : u;
+        }
+    }
+    // slot s of the lane's cell c
+    __device__ __forceinline__ long long self(int s) const { return b0 + s * ss + o[1]; }
+    // slot s of c - e_q (pull source)
+    __device__ __forceinline__ long long from(const GridDesc &g, int q, int s) const {
+        return b0 - yz_off(g, q) * YM + s * ss + o[1 + EX[q]];
+    }
+    // slot s of c + e_q (push target)
+    __device__ __forceinline__ long long to(const GridDesc &g, int q, int s) const {
+        return b0 + yz_off(g, q) * YM + s * ss + o[1 - EX[q]];
+    }
+};
 
 // e_q . u written per direction so that no product with a zero component is formed; it
 // equals the reference's ex*ux + ey*uy + ez*uz (05:288) up to the sign of a zero result,
@@ -384,6 +423,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
     const long long cb = cell_index(a.g, i0, j0, k);                  // block corner, uniform
     const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;  // lane offset
     const long long c = cb + t;
+    const Lanes<MODE> L(cb, t, N);
     S *P = static_cast<S *>(a.pop);
     const C *sx = static_cast<const C *>(a.sig_x);
     const C *sy = static_cast<const C *>(a.sig_y);
@@ -402,25 +442,25 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
         // Odd sweep with every lane active: a wave is one aligned row of 64 cells, so a
         // source row shifted by one cell in x is read as its aligned row plus one edge
         // element and moved across lanes, instead of as a load straddling two lines.
-        // signed: the left edge element sits one before the row start
-        const long long row = (long long)a.g.px * (long long)threadIdx.y;
+        // the edge elements: the cell one before the row start, or one past its end
+        const long long row = cb + (long long)a.g.px * (long long)threadIdx.y;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const long long rowoff = (long long)EY[q] * a.g.px + (long long)EZ[q] * a.g.plane;
-            const S *base = P + ((long long)INV[q] * N + cb - rowoff);
+            const long long rowoff = yz_off(a.g, q);
+            const S *base = P + (L.b0 - rowoff * L.YM + INV[q] * L.ss);   // + o[1]: slot INV[q] of c - rowoff
             if (EX[q] == 0) {
-                f[q] = ldu<MODE>(base, t, RAW);
+                f[q] = ldu<MODE>(base, L.o[1], RAW);
             } else {
-                const S v = base[t];
-                const S edge = EX[q] > 0 ? base[row - 1] : base[row + 64];
+                const S v = base[L.o[1]];
+                const S edge = P[pop_index<MODE>(INV[q], (EX[q] > 0 ? row - 1 : row + 64) - rowoff, N)];
                 f[q] = (C)wave_shift<S>(v, edge, EX[q] > 0);
             }
         }
     } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const S *src = ODD ? P + ((long long)INV[q] * N + cb - nb_off(a.g, q)) : P + ((long long)q * N + cb);
-            f[q] = ldu<MODE>(src, t, RAW);
+            const long long idx = ODD ? L.from(a.g, q, INV[q]) : L.self(q);
+            f[q] = ldu<MODE>(P + idx, 0u, RAW);
         }
     }
     // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
@@ -476,22 +516,22 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
         // -> LAY_Q: slot q of c + e_q; when c + e_q is solid, slot INV[q] of c (half-way
         // bounce-back folded into the store; the copy into the wall's slot is never read).
 #pragma unroll
-        for (int q = 0; q < Q; ++q) stu<MODE>(P + ((long long)q * N + cb + nb_off(a.g, q)), t, f[q], RAW);
+        for (int q = 0; q < Q; ++q) stu<MODE>(P + L.to(a.g, q, q), 0u, f[q], RAW);
         if (sb) {
 #pragma unroll
             for (int q = 1; q < Q; ++q)
-                if ((sb >> q) & 1u) st<MODE>(a.pop, (long long)INV[q] * N + c, f[q], RAW);
+                if ((sb >> q) & 1u) st<MODE>(a.pop, L.self(INV[q]), f[q], RAW);
         }
     } else {
         // -> LAY_P: slot INV[q] of c; a population headed into a wall is also stored,
         // reflected, in slot q of the wall cell c + e_q, where the next odd sweep reads it.
 #pragma unroll
-        for (int q = 0; q < Q; ++q) stu<MODE>(P + ((long long)INV[q] * N + cb), t, f[q], RAW);
+        for (int q = 0; q < Q; ++q) stu<MODE>(P + L.self(INV[q]), 0u, f[q], RAW);
         if (sb) {
 #pragma unroll
             for (int q = 1; q < Q; ++q)
                 if ((sb >> q) & 1u)
-                    st<MODE>(a.pop, (long long)q * N + c + nb_off(a.g, q),
+                    st<MODE>(a.pop, L.to(a.g, q, q),
                              a.alpha != 1.0 ? reflect<MODE>(q, f[q], a.alpha) : f[q], RAW);
         }
     }
@@ -598,8 +638,8 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
                 // B16: slot INV[q] of the cell itself, in its own exponent (w_INV[q] = w_q)
-                if constexpr (MODE == M_B16) bb[q] = b16_dec(cur[(long long)INV[q] * N + c], in.e[0]);
-                else bb[q] = static_cast<C>(cur[(long long)INV[q] * N + c]);
+                if constexpr (MODE == M_B16) bb[q] = b16_dec(cur[Lanes<M_B16, Q - 1>(cb, t, N).self(INV[q] - 1)], in.e[0]);
+                else bb[q] = static_cast<C>(cur[Lanes<MODE>(cb, t, N).self(INV[q])]);
             }
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
@@ -675,11 +715,11 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
         __builtin_nontemporal_store(f[0], glb(b16_rest(a.pop_out)) + cb + t);
         const auto nm = glb(b16_mants(a.pop_out, N));
 #pragma unroll
-        for (int q = 1; q < Q; ++q) __builtin_nontemporal_store(bm[q], nm + ((long long)q * N + cb) + t);
+        for (int q = 1; q < Q; ++q) __builtin_nontemporal_store(bm[q], nm + Lanes<M_B16, Q - 1>(cb, t, N).self(q - 1));
         (glb(b16_exps(a.pop_out, N)) + cb)[t] = (signed char)(be - B16_BITS);
     } else {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(static_cast<S>(f[q]), nxt + ((long long)q * N + cb) + t);
+        for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(static_cast<S>(f[q]), nxt + Lanes<MODE>(cb, t, N).self(q));
     }
 }
 
@@ -711,13 +751,13 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) in.e[q] = (ex + (cb - nb_off(a.g, q)))[t];
 #pragma unroll
-        for (int q = 1; q < Q; ++q) in.v[q] = (mn + ((long long)q * N + cb - nb_off(a.g, q)))[t];
+        for (int q = 1; q < Q; ++q) in.v[q] = mn[Lanes<M_B16, Q - 1>(cb, t, N).from(a.g, q, q - 1)];
         in.v[0] = 0;
     } else {
         const auto cur = glb(static_cast<const S *>(a.pop));
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const S v = (cur + ((long long)q * N + cb - nb_off(a.g, q)))[t];
+            const S v = cur[Lanes<MODE>(cb, t, N).from(a.g, q, q)];
             if constexpr (MODE == M_F16) {
                 if (q & 1) in.v[q >> 1].y = v;
                 else in.v[q >> 1].x = v;
@@ -770,9 +810,9 @@ __global__ __launch_bounds__(256) void k_prime_walls(void *pop, const uint8_t *m
     for (int q = 1; q < Q; ++q) {
         const long long t = c + nb_off(g, q);
         if (!(mask[t] & MK_SOLID)) continue;
-        auto v = ld<MODE>(pop, (long long)INV[q] * N + c, 1.0f);
+        auto v = ld<MODE>(pop, pop_index<MODE>(INV[q], c, N), 1.0f);
         if (alpha != 1.0) v = reflect<MODE>(q, v, alpha);
-        st<MODE>(pop, (long long)q * N + t, v, 1.0f);
+        st<MODE>(pop, pop_index<MODE>(q, t, N), v, 1.0f);
     }
 }
 
@@ -804,7 +844,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
     const uint8_t m = a.mask[c];
     if (m & MK_SOLID) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) lo[(long long)q * N + c] = C(0);
+        for (int q = 0; q < Q; ++q) lo[pop_index<MODE>(q, c, N)] = C(0);
         return;
     }
     const uint32_t sb = (m & MK_NEAR) ? a.wall_bits[c] : 0u;   // bit q: c - e_q is solid
@@ -814,10 +854,10 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
     for (int q = 0; q < Q; ++q) {
         C v;
         if ((sb >> q) & 1u) {
-            v = ld<MODE>(a.state, (long long)INV[q] * N + c, a.inv_scale);
+            v = ld<MODE>(a.state, pop_index<MODE>(INV[q], c, N), a.inv_scale);
             if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
         } else {
-            v = ld<MODE>(a.state, (long long)q * N + c - nb_off(g, q), a.inv_scale);
+            v = ld<MODE>(a.state, pop_index<MODE>(q, c - nb_off(g, q), N), a.inv_scale);
         }
         if constexpr (MODE != M_F64) v += wq<C>(q);   // deviation storage
         f[q] = v;
@@ -839,7 +879,7 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
     C A = C(0), Bx = C(0), By = C(0), Bz = C(0), Cx = C(0), Cy = C(0), Cz = C(0), D = C(0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        lam[q] = li[(long long)q * N + c];
+        lam[q] = li[pop_index<MODE>(q, c, N)];
         const C wl = wq<C>(q) * lam[q];
         const C eu = (C)EX[q] * ux + (C)EY[q] * uy + (C)EZ[q] * uz;
         A += wl;
@@ -861,17 +901,17 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
     for (int q = 0; q < Q; ++q) {
         const C mu = keep * ((C(1) - om) * lam[q] + om * (base + (C)EX[q] * vx + (C)EY[q] * vy + (C)EZ[q] * vz)) + probe;
         if ((sb >> q) & 1u) {
-            lo[(long long)INV[q] * N + c] = (C)a.alpha * mu;
+            lo[pop_index<MODE>(INV[q], c, N)] = (C)a.alpha * mu;
         } else {
             const int si = i - EX[q], sj = j - EY[q], sk = k - EZ[q];
             if (si >= 1 && si <= g.nx && sj >= 1 && sj <= g.ny && sk >= 1 && sk <= g.nz)
-                lo[(long long)q * N + c - nb_off(g, q)] = mu;
+                lo[pop_index<MODE>(q, c - nb_off(g, q), N)] = mu;
         }
         // slot q of c is read only by the halo cell c + e_q (a constant): zero it
         const int ti = i + EX[q], tj = j + EY[q], tk = k + EZ[q];
         if (q != 0 && (ti < 1 || ti > g.nx || tj < 1 || tj > g.ny || tk < 1 || tk > g.nz) &&
             !(a.mask[c + nb_off(g, q)] & MK_SOLID))
-            lo[(long long)q * N + c] = C(0);
+            lo[pop_index<MODE>(q, c, N)] = C(0);
     }
 }
 
@@ -890,7 +930,7 @@ __global__ __launch_bounds__(256) void k_source_dot(const void *lam, long long n
     if (e >= n) return;
     const C *l = static_cast<const C *>(lam);
     double acc = 0.0;
-    for (int q = 0; q < Q; ++q) acc += W64[q] * (double)l[(long long)q * ncell + cells[e]];
+    for (int q = 0; q < Q; ++q) acc += W64[q] * (double)l[pop_index<MODE>(q, cells[e], ncell)];
     out[e] = acc;
 }
 
@@ -905,12 +945,12 @@ __device__ __forceinline__ typename Tr<MODE>::C read_post(const void *pop, const
         static_assert(LAY == LAY_AB, "B16 runs with the two-array pattern only");
         return b16_slot_g(pop, N, c, q);
     } else if constexpr (LAY == LAY_AB) {
-        return ld<MODE>(pop, (long long)q * N + c, inv_scale);
+        return ld<MODE>(pop, pop_index<MODE>(q, c, N), inv_scale);
     } else if constexpr (LAY == LAY_P) {
-        return ld<MODE>(pop, (long long)INV[q] * N + c, inv_scale);
+        return ld<MODE>(pop, pop_index<MODE>(INV[q], c, N), inv_scale);
     } else {
         const long long t = c + nb_off(g, q);
-        const long long idx = (mask[t] & MK_SOLID) ? (long long)INV[q] * N + c : (long long)q * N + t;
+        const long long idx = (mask[t] & MK_SOLID) ? pop_index<MODE>(INV[q], c, N) : pop_index<MODE>(q, t, N);
         return ld<MODE>(pop, idx, inv_scale);
     }
 }
@@ -995,9 +1035,9 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
             // neighbour, slot q of I + e_q (LAY_Q)
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                fI[q] = ld<MODE>(a.pop, LAY == LAY_AB ? (long long)q * g.ncell + I
-                                        : LAY == LAY_P ? (long long)INV[q] * g.ncell + I
-                                                       : (long long)q * g.ncell + I + nb_off(g, q), a.inv_scale);
+                fI[q] = ld<MODE>(a.pop, LAY == LAY_AB ? pop_index<MODE>(q, I, N)
+                                        : LAY == LAY_P ? pop_index<MODE>(INV[q], I, N)
+                                                       : pop_index<MODE>(q, I + nb_off(g, q), N), a.inv_scale);
         } else {
 #pragma unroll
             for (int q = 0; q < Q; ++q) fI[q] = read_post<MODE, LAY>(a.pop, a.mask, g, I, q, a.inv_scale);
@@ -1074,11 +1114,11 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
             const long long tc = h + nb_off(g, q);
             long long idx;
             if constexpr (LAY == LAY_AB) {
-                idx = (long long)q * N + h;
+                idx = pop_index<MODE>(q, h, N);
             } else if constexpr (LAY == LAY_P) {
-                idx = (long long)INV[q] * N + h;
+                idx = pop_index<MODE>(INV[q], h, N);
             } else {
-                idx = (a.mask[tc] & MK_SOLID) ? (long long)INV[q] * N + h : (long long)q * N + tc;
+                idx = (a.mask[tc] & MK_SOLID) ? pop_index<MODE>(INV[q], h, N) : pop_index<MODE>(q, tc, N);
             }
             st<MODE>(a.pop, idx, val, a.scale);
         }
@@ -1129,7 +1169,7 @@ __device__ __forceinline__ void solid_body(const SolidArgs &a, int n) {
         } else if constexpr (MODE == M_B16) {
             v = b16_slot_g(a.pop, N, src, q);
         } else {
-            v = ld<MODE>(a.pop, (long long)(a.lay == LAY_AB ? q : INV[q]) * N + src, a.inv_scale);
+            v = ld<MODE>(a.pop, pop_index<MODE>(a.lay == LAY_AB ? q : INV[q], src, N), a.inv_scale);
         }
         sn[(long long)n * Q + q] = v;
     }
@@ -1151,16 +1191,16 @@ __global__ __launch_bounds__(256) void k_solid_multi(const SolidArgs *__restrict
 // ---- source injection (05:660-665) ---------------------------------------------------------
 template <int MODE, int LAY>
 __device__ __forceinline__ void source_body(const SourceArgs &a, int e, int q) {
-    const long long c = a.cells[e];
     const long long N = a.g.ncell;
+    const long long c = a.cells[e];
     long long idx;
     if constexpr (LAY == LAY_AB) {
-        idx = (long long)q * N + c;
+        idx = pop_index<MODE>(q, c, N);
     } else if constexpr (LAY == LAY_P) {
-        idx = (long long)INV[q] * N + c;
+        idx = pop_index<MODE>(INV[q], c, N);
     } else {
         const long long tc = c + nb_off(a.g, q);
-        idx = (a.mask[tc] & MK_SOLID) ? (long long)INV[q] * N + c : (long long)q * N + tc;
+        idx = (a.mask[tc] & MK_SOLID) ? pop_index<MODE>(INV[q], c, N) : pop_index<MODE>(q, tc, N);
     }
     typename Tr<MODE>::C v;
     if constexpr (MODE == M_F64) {
@@ -1175,7 +1215,7 @@ __device__ __forceinline__ void source_body(const SourceArgs &a, int e, int q) {
         // keep the wall copy of a population headed into a wall in step (see Layout)
         const long long tc = c + nb_off(a.g, q);
         if (q != 0 && (a.mask[tc] & MK_SOLID))
-            st<MODE>(a.pop, (long long)q * N + tc, a.alpha != 1.0 ? reflect<MODE>(q, v, a.alpha) : v, a.scale);
+            st<MODE>(a.pop, pop_index<MODE>(q, tc, N), a.alpha != 1.0 ? reflect<MODE>(q, v, a.alpha) : v, a.scale);
     }
     // the x-face buffer holds the same f*_q of boundary cells; apply the same add to it
     const long long r = c - a.g.origin;
@@ -1282,12 +1322,12 @@ __global__ __launch_bounds__(256) void k_fill_rest(void *pop, long long ncell) {
     if constexpr (MODE == M_B16) {
         b16_rest(pop)[t] = 0.0f;
 #pragma unroll
-        for (int q = 1; q < Q; ++q) b16_mants(pop, ncell)[(long long)q * ncell + t] = 0;
+        for (int q = 1; q < Q; ++q) b16_mants(pop, ncell)[mant_index(q, t)] = 0;
         b16_exps(pop, ncell)[t] = 0;
         return;
     } else {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) st<MODE>(pop, (long long)q * ncell + t, rest<MODE>(q), 1.0f);
+        for (int q = 0; q < Q; ++q) st<MODE>(pop, pop_index<MODE>(q, t, ncell), rest<MODE>(q), 1.0f);
     }
 }
 
@@ -1314,8 +1354,8 @@ __global__ __launch_bounds__(256) void k_set_density(void *pop, const uint8_t *m
     for (int q = 0; q < Q; ++q) {
         // feq(q, rho, 0, 0, 0) = w_q * rho (05:286-291 with u = 0); symmetric in q, so the
         // value is the same in either slot convention.
-        if constexpr (MODE == M_F64) st<MODE>(pop, (long long)q * g.ncell + c, W64[q] * r, scale);
-        else if constexpr (MODE != M_B16) st<MODE>(pop, (long long)q * g.ncell + c, (float)(W64[q] * (r - 1.0)), scale);
+        if constexpr (MODE == M_F64) st<MODE>(pop, pop_index<MODE>(q, c, g.ncell), W64[q] * r, scale);
+        else if constexpr (MODE != M_B16) st<MODE>(pop, pop_index<MODE>(q, c, g.ncell), (float)(W64[q] * (r - 1.0)), scale);
     }
 }
 

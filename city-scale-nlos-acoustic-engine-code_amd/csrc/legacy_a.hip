@@ -9,7 +9,7 @@
 //   k_a_stream    interior fluid cells: pull the post-collision value, or bounce the cell's
 //                 own pre-collision value back when the source is solid, blend the sponge
 //                 of the destination, rho = sum in slot order (01:308-356)
-// Populations are SoA in ordering A (01:17-19), f[q*ncell + cell], cell as in GridDesc.
+// Populations in ordering A (01:17-19), slot-major as every FP64 array: f[q*ncell + cell], cell as in GridDesc.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_a_halo_xy(double *f, GridDesc g) {
     const long long c = cell_index(g, i, j, k);
     // feq(q, 1, 0, 0, 0) = w_q exactly (every term of the bracket but 1 is zero)
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[(long long)q * g.ncell + c] = W64[q];
+    for (int q = 0; q < Q; ++q) f[pop_index<M_F64>(q, c, g.ncell)] = W64[q];
 }
 
 __global__ __launch_bounds__(256) void k_a_halo_z(double *f, GridDesc g) {
@@ -63,9 +63,8 @@ __global__ __launch_bounds__(256) void k_a_halo_z(double *f, GridDesc g) {
     const long long top = cell_index(g, i, j, g.nz), bot = cell_index(g, i, j, 1);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const long long o = (long long)q * g.ncell;
-        f[o + lo] = f[o + top];
-        f[o + hi] = f[o + bot];
+        f[pop_index<M_F64>(q, lo, g.ncell)] = f[pop_index<M_F64>(q, top, g.ncell)];
+        f[pop_index<M_F64>(q, hi, g.ncell)] = f[pop_index<M_F64>(q, bot, g.ncell)];
     }
 }
 
@@ -81,7 +80,7 @@ __global__ __launch_bounds__(256) void k_a_collide(const double *f, double *post
     if (mask[c] & MK_SOLID) return;
     double v[Q];
 #pragma unroll
-    for (int p = 0; p < Q; ++p) v[p] = f[(long long)p * g.ncell + c];
+    for (int p = 0; p < Q; ++p) v[p] = f[pop_index<M_F64>(p, c, g.ncell)];
     double sr = 0.0, sux = 0.0, suy = 0.0, suz = 0.0;
 #pragma unroll
     for (int p = 0; p < Q; ++p) {
@@ -97,7 +96,7 @@ __global__ __launch_bounds__(256) void k_a_collide(const double *f, double *post
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const double fe = feq_a(q, sr, sux, suy, suz);
-        post[(long long)q * g.ncell + c] = v[q] - (v[q] - fe) / tau;
+        post[pop_index<M_F64>(q, c, g.ncell)] = v[q] - (v[q] - fe) / tau;
     }
 }
 
@@ -122,16 +121,16 @@ __global__ __launch_bounds__(256) void k_a_stream(AStepArgs a) {
     for (int q = 0; q < Q; ++q) {
         const long long src = c - off_a(g, q);
         if (a.mask[src] & MK_SOLID) {
-            out[q] = a.f[(long long)INVA[q] * g.ncell + c];
+            out[q] = a.f[pop_index<M_F64>(INVA[q], c, g.ncell)];
         } else {
-            const double p = a.post[(long long)q * g.ncell + src];
+            const double p = a.post[pop_index<M_F64>(q, src, g.ncell)];
             out[q] = in_sponge ? (1.0 - s) * p + s * W64[q] : p;   // feq(q, 1, 0, 0, 0) = w_q
         }
     }
     double sum = 0.0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        a.f[(long long)q * g.ncell + c] = out[q];
+        a.f[pop_index<M_F64>(q, c, g.ncell)] = out[q];
         sum += out[q];
     }
     if (!isfinite(sum)) atomicOr(a.flag, 1);

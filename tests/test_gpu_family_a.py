@@ -168,4 +168,4 @@ def test_checkpoint_of_the_other_family_is_refused(gpu):
     b = _engine(cfg_b, solid, None, None, pts)
     with pytest.raises(nlos.EngineError) as ex:
         b.restore(blob)
-    assert ex.value.code == 2
+    assert ex.value.code == 1                    # NLOS_ERR_ARG
