@@ -125,6 +125,7 @@ struct StepArgs {
     int dpp;                             // every lane active: odd sweep may shift across lanes
     void *fb[2];                         // x-face buffers (x-, x+) or null, see below
     const uint32_t *wall_bits;           // two-array sweep: bit q set when c - e_q is solid
+    const void *xh;                      // x-halo records or null, see below
 };
 
 // x-face buffers.  The cells I = (1, j, k) and (nx, j, k) feed the NSCBC ghosts of the x
@@ -133,6 +134,31 @@ struct StepArgs {
 // to fb[face][(j-1) + ny*(k-1)][q]; the source kernel adds to them like to the lattice, and
 // the ghost kernel reads them contiguously.  B16 records are FP32 deviations g equal to the
 // decoded stored values.
+
+// x-halo records (blocked storage, two-array pattern).  The halo cells i = 0 and i = nx + 1
+// sit alone in their 64-cell blocks, so storing the ghost values there and pulling them
+// back moves a whole line per slot for one value each way: 85 of the ghost kernel's 111 us
+// per step at 1024x1024x256 went to those stores.  When xh is set, the five slots that
+// interior cells pull from such a cell (EX = +1 at i = 0, EX = -1 at i = nx + 1) are kept in
+// xh instead, in storage form, rows (j, k) over the whole halo ring [0, ny+1] x [0, nz+1]:
+// slot q of the cell at xh_index (FP16: the scaled half; B16: the 16-bit mantissa, and the
+// cell's exponent byte at xh_exp_index).  The ghost kernel writes them and the sweep and
+// the solid kernel read them, within one step; the lattice slots of those cells are not
+// written and not read.
+constexpr int XR[Q] = {-1, 0, 0, -1, -1, -1, -1, 1, 2, 1, 2, 3, 4, 3, 4, -1, -1, -1, -1};   // rank among the 5 slots of one EX
+__host__ __device__ __forceinline__ long long xh_rows(const GridDesc &g) {
+    return (long long)(g.ny + 2) * (g.nz + 2);
+}
+// slot q (EX[q] != 0) of the halo cell (0, j, k) if EX[q] = +1, (nx + 1, j, k) if EX[q] = -1
+__host__ __device__ __forceinline__ long long xh_index(const GridDesc &g, int q, int j, int k) {
+    return (long long)((EX[q] > 0 ? 0 : 5) + XR[q]) * xh_rows(g) + j + (long long)(g.ny + 2) * k;
+}
+// byte offset of the B16 exponent of the halo cell on face f (0: i = 0, 1: i = nx + 1)
+__host__ __device__ __forceinline__ long long xh_exp_offset(const GridDesc &g, int f, int j, int k) {
+    return 20 * xh_rows(g) + f * xh_rows(g) + j + (long long)(g.ny + 2) * k;
+}
+// bytes of the record buffer
+__host__ __device__ __forceinline__ long long xh_bytes(const GridDesc &g) { return 22 * xh_rows(g); }
 
 struct GhostArgs {
     void *pop;
@@ -144,6 +170,7 @@ struct GhostArgs {
     const int *face_map[6];               // per NSCBC face: plane index -> state slot or -1
     float scale, inv_scale;
     long long n_halo;
+    void *xh;                             // x-halo records or null (see StepArgs)
 };
 
 struct SolidArgs {
@@ -157,6 +184,7 @@ struct SolidArgs {
     void *s_new;
     double alpha;
     float scale, inv_scale;
+    const void *xh;                       // x-halo records or null (see StepArgs)
 };
 
 struct SourceArgs {

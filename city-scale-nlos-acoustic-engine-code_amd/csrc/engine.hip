@@ -231,6 +231,7 @@ struct nlos_engine {
 
     DevBuf pop, pop2, mask, sig[3], flag, field, hist, probe_cells, s_a, s_b, solid_cells, src_cells,
         src_adds, face_map[6], gh, vox_tri, vox_bins, vox_start, rho_tmp, fb[2], wall_bits;
+    DevBuf xh;                        // x-halo records of the blocked two-array modes (StepArgs::xh)
     bool fb_valid = false;           // x-face buffers agree with the lattice
     bool walls_valid = false;        // LAY_P wall copies agree with the lattice
     bool field_valid = false;
@@ -656,6 +657,7 @@ GhostArgs ghost_args(const nlos_engine *e, int phase) {
     a.scale = e->scale;
     a.inv_scale = e->inv_scale;
     a.n_halo = halo_cells(e->g);
+    a.xh = e->xh.p;
     return a;
 }
 
@@ -672,6 +674,7 @@ SolidArgs solid_args(const nlos_engine *e, int phase) {
     s.alpha = e->cfg.wall_reflect;
     s.scale = e->scale;
     s.inv_scale = e->inv_scale;
+    s.xh = e->xh.p;
     return s;
 }
 
@@ -700,6 +703,7 @@ StepArgs step_args(const nlos_engine *e, int phase, dim3 block) {
     a.fb[0] = e->fb[0].p;
     a.fb[1] = e->fb[1].p;
     a.wall_bits = e->wall_bits.as<uint32_t>();
+    a.xh = e->xh.p;
     return a;
 }
 
@@ -971,6 +975,14 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate a second %.2f GB population array", popb / 1e9));
     if (e->pattern == NLOS_PATTERN_TWO_ARRAY && e->wall_bits.reserve((size_t)e->g.ncell * 4) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate wall bits"));
+    // x-halo records; NLOS_XHALO=0 keeps those halo values in the lattice (the same results)
+    const char *xh_env = std::getenv("NLOS_XHALO");
+    if (e->pattern == NLOS_PATTERN_TWO_ARRAY && blocked_mode(e->mode) && !(xh_env && std::atoi(xh_env) == 0)) {
+        if (e->xh.reserve((size_t)xh_bytes(e->g)) != hipSuccess)
+            return bail(fail(NLOS_ERR_NOMEM, "cannot allocate x-halo records"));
+        if (hipMemset(e->xh.p, 0, (size_t)xh_bytes(e->g)) != hipSuccess)
+            return bail(fail(NLOS_ERR_HIP, "memset x-halo records"));
+    }
     if (e->mask.reserve((size_t)e->g.ncell) != hipSuccess || e->flag.reserve(sizeof(int)) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
     for (int f = 0; f < 2; ++f)
@@ -1017,7 +1029,7 @@ NLOS_API int nlos_destroy(nlos_engine *e) {
                       &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
                       &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
                       &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1], &e->wall_bits,
-                      &e->sigma_a, &e->field_a};
+                      &e->sigma_a, &e->field_a, &e->xh};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : e->ck) b.release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
@@ -1042,7 +1054,7 @@ NLOS_API int nlos_get_info(const nlos_engine *e, nlos_info *info) {
     const DevBuf *bufs[] = {&e->pop, &e->pop2, &e->mask, &e->sig[0], &e->sig[1], &e->sig[2], &e->flag, &e->field,
                             &e->hist, &e->probe_cells, &e->s_a, &e->s_b, &e->solid_cells, &e->src_cells,
                             &e->src_adds, &e->gh, &e->fb[0], &e->fb[1], &e->wall_bits, &e->sigma_a,
-                            &e->field_a};
+                            &e->field_a, &e->xh};
     for (const DevBuf *b : bufs) tot += b->bytes;
     info->device_bytes = tot;
     return NLOS_OK;

@@ -744,20 +744,35 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;
     const long long c = cb + t;
     AbIn<MODE> in;
+    // x-halo sources (see StepArgs::xh): slot q of (i - EX[q], j', k') with i - EX[q] = 0 or
+    // nx + 1 comes from the record of row (j', k') instead of the lattice
+    const bool xlo = a.xh && i == 1, xhi = a.xh && i == a.g.nx;
+    const auto xsrc = [&](int q) { return EX[q] != 0 && (EX[q] > 0 ? xlo : xhi); };
     if constexpr (MODE == M_B16) {
         const auto ex = glb(b16_exps(a.pop, N));
         const auto mn = glb(b16_mants(a.pop, N));
+        const auto xm = glb(static_cast<const short *>(a.xh));
+        const auto xe = glb(static_cast<const signed char *>(a.xh));
         in.r0 = (glb(b16_rest(a.pop)) + cb)[t];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) in.e[q] = (ex + (cb - nb_off(a.g, q)))[t];
+        for (int q = 0; q < Q; ++q) {
+            if (xsrc(q)) in.e[q] = xe[xh_exp_offset(a.g, EX[q] > 0 ? 0 : 1, j - EY[q], k - EZ[q])];
+            else in.e[q] = (ex + (cb - nb_off(a.g, q)))[t];
+        }
 #pragma unroll
-        for (int q = 1; q < Q; ++q) in.v[q] = mn[Lanes<M_B16, Q - 1>(cb, t, N).from(a.g, q, q - 1)];
+        for (int q = 1; q < Q; ++q) {
+            if (xsrc(q)) in.v[q] = xm[xh_index(a.g, q, j - EY[q], k - EZ[q])];
+            else in.v[q] = mn[Lanes<M_B16, Q - 1>(cb, t, N).from(a.g, q, q - 1)];
+        }
         in.v[0] = 0;
     } else {
         const auto cur = glb(static_cast<const S *>(a.pop));
+        [[maybe_unused]] const auto xs = glb(static_cast<const S *>(a.xh));
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const S v = cur[Lanes<MODE>(cb, t, N).from(a.g, q, q)];
+            S v;
+            if (blocked_mode(MODE) && xsrc(q)) v = xs[xh_index(a.g, q, j - EY[q], k - EZ[q])];
+            else v = cur[Lanes<MODE>(cb, t, N).from(a.g, q, q)];
             if constexpr (MODE == M_F16) {
                 if (q & 1) in.v[q >> 1].y = v;
                 else in.v[q >> 1].x = v;
@@ -1087,6 +1102,8 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
 
     C gout[Q];            // B16: the cell is coded once, after the loop
     unsigned pulled = 0;  // B16: slots that interior cells pull
+    // x-halo cell whose pulled slots go to the records (see StepArgs::xh)
+    const bool xcell = blocked_mode(MODE) && LAY == LAY_AB && a.xh && (i == 0 || i == nx + 1);
 #pragma unroll
     for (int q = 0; q < Q; ++q) gout[q] = rest<MODE>(q);
 #pragma unroll
@@ -1111,6 +1128,10 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
             pulled |= 1u << q;
             continue;
         } else {
+            if (xcell) {
+                st<MODE>(a.xh, xh_index(g, q, j, k), val, a.scale);
+                continue;
+            }
             const long long tc = h + nb_off(g, q);
             long long idx;
             if constexpr (LAY == LAY_AB) {
@@ -1127,7 +1148,18 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
         // slots no interior cell pulls are never read: only the pulled ones are written
         float x[Q];
         b16_g_to_x(gout, x);
-        b16_store_x(a.pop, N, h, x, pulled);
+        if (xcell) {
+            // the coding of b16_store_x, into the records
+            const int e = b16_exp_of(x);
+            short m[Q];
+            b16_code(x, e, m);
+#pragma unroll
+            for (int q = 1; q < Q; ++q)
+                if ((pulled >> q) & 1u) static_cast<short *>(a.xh)[xh_index(g, q, j, k)] = m[q];
+            static_cast<signed char *>(a.xh)[xh_exp_offset(g, i == 0 ? 0 : 1, j, k)] = (signed char)(e - B16_BITS);
+        } else {
+            b16_store_x(a.pop, N, h, x, pulled);
+        }
     }
 }
 
@@ -1163,9 +1195,26 @@ __device__ __forceinline__ void solid_body(const SolidArgs &a, int n) {
     for (int q = 0; q < Q; ++q) {
         const long long src = c - nb_off(a.g, q);
         C v;
+        // an x-halo source keeps this slot in the records (see StepArgs::xh)
+        bool xsrc = false;
+        int sj = 0, sk = 0;
+        if (blocked_mode(MODE) && a.xh && EX[q] != 0) {
+            const long long r = src - a.g.origin;
+            const int si = (int)(r % a.g.px);
+            sj = (int)((r / a.g.px) % a.g.py);
+            sk = (int)(r / a.g.plane);
+            xsrc = EX[q] > 0 ? si == 0 : si == a.g.nx + 1;
+        }
         if (a.mask[src] & MK_SOLID) {
             v = so[(long long)n * Q + INV[q]];
             if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
+        } else if (xsrc) {
+            if constexpr (MODE == M_B16) {
+                const int E = static_cast<const signed char *>(a.xh)[xh_exp_offset(a.g, EX[q] > 0 ? 0 : 1, sj, sk)];
+                v = b16_dec(static_cast<const short *>(a.xh)[xh_index(a.g, q, sj, sk)], E) * wq<float>(q);
+            } else {
+                v = ld<MODE>(a.xh, xh_index(a.g, q, sj, sk), a.inv_scale);
+            }
         } else if constexpr (MODE == M_B16) {
             v = b16_slot_g(a.pop, N, src, q);
         } else {
