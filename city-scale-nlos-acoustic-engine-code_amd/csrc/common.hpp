@@ -135,16 +135,16 @@ struct StepArgs {
 // the ghost kernel reads them contiguously.  B16 records are FP32 deviations g equal to the
 // decoded stored values.
 
-// x-halo records (blocked storage, two-array pattern).  The halo cells i = 0 and i = nx + 1
-// sit alone in their 64-cell blocks, so storing the ghost values there and pulling them
-// back moves a whole line per slot for one value each way: 85 of the ghost kernel's 111 us
-// per step at 1024x1024x256 went to those stores.  When xh is set, the five slots that
-// interior cells pull from such a cell (EX = +1 at i = 0, EX = -1 at i = nx + 1) are kept in
-// xh instead, in storage form, rows (j, k) over the whole halo ring [0, ny+1] x [0, nz+1]:
-// slot q of the cell at xh_index (FP16: the scaled half; B16: the 16-bit mantissa, and the
-// cell's exponent byte at xh_exp_index).  The ghost kernel writes them and the sweep and
-// the solid kernel read them, within one step; the lattice slots of those cells are not
-// written and not read.
+// x-halo records (two-array pattern).  The halo cells i = 0 and i = nx + 1 sit alone in
+// their cache lines (their 64-cell blocks in the 16-bit modes), so storing the ghost values
+// there and pulling them back moves a whole line per slot for one value each way: 85 of
+// the FP16 ghost kernel's 111 us per step at 1024x1024x256 went to those stores.  When xh
+// is set, the five slots that interior cells pull from such a cell (EX = +1 at i = 0,
+// EX = -1 at i = nx + 1) are kept in xh instead, in storage form, rows (j, k) over the whole
+// halo ring [0, ny+1] x [0, nz+1]: slot q of the cell at element xh_index of the storage
+// type (B16: the 16-bit mantissa, with the cell's exponent byte at xh_exp_offset).  The
+// ghost kernel writes them and the sweep and the solid kernel read them, within one step;
+// the lattice slots of those cells are not written and not read.
 constexpr int XR[Q] = {-1, 0, 0, -1, -1, -1, -1, 1, 2, 1, 2, 3, 4, 3, 4, -1, -1, -1, -1};   // rank among the 5 slots of one EX
 __host__ __device__ __forceinline__ long long xh_rows(const GridDesc &g) {
     return (long long)(g.ny + 2) * (g.nz + 2);
@@ -157,8 +157,10 @@ __host__ __device__ __forceinline__ long long xh_index(const GridDesc &g, int q,
 __host__ __device__ __forceinline__ long long xh_exp_offset(const GridDesc &g, int f, int j, int k) {
     return 20 * xh_rows(g) + f * xh_rows(g) + j + (long long)(g.ny + 2) * k;
 }
-// bytes of the record buffer
-__host__ __device__ __forceinline__ long long xh_bytes(const GridDesc &g) { return 22 * xh_rows(g); }
+// bytes of the record buffer for elements of s_bytes bytes
+__host__ __device__ __forceinline__ long long xh_bytes(const GridDesc &g, int mode, int s_bytes) {
+    return 10 * s_bytes * xh_rows(g) + (mode == M_B16 ? 2 * xh_rows(g) : 0);
+}
 
 struct GhostArgs {
     void *pop;

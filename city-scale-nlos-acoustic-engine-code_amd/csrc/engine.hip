@@ -977,11 +977,10 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate wall bits"));
     // x-halo records; NLOS_XHALO=0 keeps those halo values in the lattice (the same results)
     const char *xh_env = std::getenv("NLOS_XHALO");
-    if (e->pattern == NLOS_PATTERN_TWO_ARRAY && blocked_mode(e->mode) && !(xh_env && std::atoi(xh_env) == 0)) {
-        if (e->xh.reserve((size_t)xh_bytes(e->g)) != hipSuccess)
-            return bail(fail(NLOS_ERR_NOMEM, "cannot allocate x-halo records"));
-        if (hipMemset(e->xh.p, 0, (size_t)xh_bytes(e->g)) != hipSuccess)
-            return bail(fail(NLOS_ERR_HIP, "memset x-halo records"));
+    if (e->pattern == NLOS_PATTERN_TWO_ARRAY && !fam_a && !(xh_env && std::atoi(xh_env) == 0)) {
+        const size_t xb = (size_t)xh_bytes(e->g, e->mode, e->s_bytes);
+        if (e->xh.reserve(xb) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "cannot allocate x-halo records"));
+        if (hipMemset(e->xh.p, 0, xb) != hipSuccess) return bail(fail(NLOS_ERR_HIP, "memset x-halo records"));
     }
     if (e->mask.reserve((size_t)e->g.ncell) != hipSuccess || e->flag.reserve(sizeof(int)) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
@@ -2323,14 +2322,18 @@ NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_
             sa.cells = src_cells.as<long long>() + off[k];
             sa.adds = src_adds.as<double>() + off[k];
             sa.n = (int)(off[k + 1] - off[k]);
+            // the adjoint reads these states whole, x-halo cells included: the replay keeps
+            // the halo values in the lattice (no x-halo records)
             GhostArgs ga = ghost_args(e, 0);
             ga.pop = store[k].p;
             ga.fb[0] = ga.fb[1] = nullptr;
+            ga.xh = nullptr;
             StepArgs st = step_args(e, 0, e->block);
             st.pop = store[k].p;
             st.pop_out = store[k + 1].p;
             st.n_probe_cells = 0;
             st.fb[0] = st.fb[1] = nullptr;
+            st.xh = nullptr;
             rc = hip(by_mode(e->mode, [&](auto L) {
                 using LL = decltype(L);
                 hipError_t r = LL::source(sa, LAY_AB, e->stream);

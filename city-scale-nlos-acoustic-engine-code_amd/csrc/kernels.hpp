@@ -767,11 +767,11 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
         in.v[0] = 0;
     } else {
         const auto cur = glb(static_cast<const S *>(a.pop));
-        [[maybe_unused]] const auto xs = glb(static_cast<const S *>(a.xh));
+        const auto xs = glb(static_cast<const S *>(a.xh));
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             S v;
-            if (blocked_mode(MODE) && xsrc(q)) v = xs[xh_index(a.g, q, j - EY[q], k - EZ[q])];
+            if (xsrc(q)) v = xs[xh_index(a.g, q, j - EY[q], k - EZ[q])];
             else v = cur[Lanes<MODE>(cb, t, N).from(a.g, q, q)];
             if constexpr (MODE == M_F16) {
                 if (q & 1) in.v[q >> 1].y = v;
@@ -1103,7 +1103,7 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
     C gout[Q];            // B16: the cell is coded once, after the loop
     unsigned pulled = 0;  // B16: slots that interior cells pull
     // x-halo cell whose pulled slots go to the records (see StepArgs::xh)
-    const bool xcell = blocked_mode(MODE) && LAY == LAY_AB && a.xh && (i == 0 || i == nx + 1);
+    const bool xcell = LAY == LAY_AB && a.xh && (i == 0 || i == nx + 1);
 #pragma unroll
     for (int q = 0; q < Q; ++q) gout[q] = rest<MODE>(q);
 #pragma unroll
@@ -1198,7 +1198,7 @@ __device__ __forceinline__ void solid_body(const SolidArgs &a, int n) {
         // an x-halo source keeps this slot in the records (see StepArgs::xh)
         bool xsrc = false;
         int sj = 0, sk = 0;
-        if (blocked_mode(MODE) && a.xh && EX[q] != 0) {
+        if (a.xh && EX[q] != 0) {
             const long long r = src - a.g.origin;
             const int si = (int)(r % a.g.px);
             sj = (int)((r / a.g.px) % a.g.py);

@@ -1,6 +1,6 @@
-"""x-halo records (FP16S and B16, two-array pattern).
+"""x-halo records (two-array pattern, every storage mode).
 
-The halo cells i = 0 and i = nx + 1 sit alone in their 64-cell blocks of the lattice, so the
+The halo cells i = 0 and i = nx + 1 sit alone in their cache lines of the lattice, so the
 engine keeps the five slots that interior cells pull from each of them in compact per-row
 records: the ghost kernel writes them; the sweep and the solid kernel read them.  With
 NLOS_XHALO=0 the same values go through the lattice instead.  Both ways must give the same
@@ -50,7 +50,7 @@ def _run(cfg, solid, src, rho, probes, steps):
     return out
 
 
-@pytest.mark.parametrize("precision", ["f16s", "b16"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f16s", "b16"])
 @pytest.mark.parametrize("scene", ["box", "box128", "pulse", "reflect"])
 def test_xhalo_records_equal_lattice_halo(gpu, precision, scene, monkeypatch):
     print(f"[x-halo records] scene={scene} precision={precision}", flush=True)
