@@ -145,7 +145,8 @@ struct StepArgs {
 // type (B16: the 16-bit mantissa, with the cell's exponent byte at xh_exp_offset).  The
 // ghost kernel writes them and the sweep and the solid kernel read them, within one step;
 // the lattice slots of those cells are not written and not read.
-constexpr int XR[Q] = {-1, 0, 0, -1, -1, -1, -1, 1, 2, 1, 2, 3, 4, 3, 4, -1, -1, -1, -1};   // rank among the 5 slots of one EX
+// Rank of slot q among the five slots with the same nonzero EX[q] (-1 where EX[q] = 0).
+constexpr int XR[Q] = {-1, 0, 0, -1, -1, -1, -1, 1, 2, 1, 2, 3, 4, 3, 4, -1, -1, -1, -1};
 __host__ __device__ __forceinline__ long long xh_rows(const GridDesc &g) {
     return (long long)(g.ny + 2) * (g.nz + 2);
 }

@@ -542,8 +542,9 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a) {
 // nxt[q][c] (aligned, local).  Solid lanes store the rest state into their own slots,
 // which nothing reads: every wave then writes whole 128-byte lines, where a line with
 // unwritten lanes costs the memory a read-modify-write.  One thread per cell, one plane
-// per workgroup tile, plain per-lane addressing (the compiler keeps every slot offset as a
-// constant off one cell index).
+// per workgroup tile, per-lane addressing through Lanes (a workgroup-uniform base plus a
+// 32-bit lane offset; the compiler keeps every slot offset as a constant).  Pulls from
+// the x-halo cells come from the x-halo records when StepArgs::xh is set.
 
 template <class S>
 __device__ __forceinline__ S read_lane(S v, int src) {
