@@ -273,7 +273,9 @@ int nlos_write_probe_dat(nlos_engine *e, const char *dir, const char *const *nam
 
 /* ---- checkpoint / restore ------------------------------------------------------------ */
 /* Serialises populations, tracked boundary-solid state, step index and probe histories.
- * With buf == NULL only *needed is set. */
+ * With buf == NULL only *needed is set.  Halo (ghost) values are not part of the state: a
+ * caller that drives the phases one by one calls nlos_exchange_halos after nlos_restore,
+ * before nlos_step_collision, as every reference step does (nlos_step always does). */
 int nlos_checkpoint(nlos_engine *e, void *buf, size_t cap, size_t *needed);
 int nlos_restore(nlos_engine *e, const void *buf, size_t len);
 
