@@ -965,9 +965,12 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     if (e->mode == M_B16 && e->pattern != NLOS_PATTERN_TWO_ARRAY)
         return bail(fail(NLOS_ERR_NOMEM, "NLOS_B16 needs two population arrays of %.2f GB each", popb / 1e9));
     e->lay = e->pattern == NLOS_PATTERN_TWO_ARRAY ? LAY_AB : LAY_P;
-    // Measured on the 1024x1024x256 city: the pull sweep runs fastest on 256-cell row
-    // segments, the in-place sweep on 64x4 tiles (its lane shifts need 64-cell rows).
-    if (e->pattern == NLOS_PATTERN_TWO_ARRAY) e->block = dim3(256, 1, 1);
+    // Measured on the 1024x1024x256 city: both sweeps run fastest on 256-cell row segments
+    // (in place: F16S 5.18 -> 4.74 ms, FP32 10.31 -> 9.50 ms against 64x4 tiles, whose
+    // x-shifted stores of the odd sweep half-fill lines that a workgroup on another XCD
+    // fills).  The odd sweep's lane shifts need 64x4 tiles (NLOS_BLOCK=64x4); with them it
+    // measured the same as plain shifted loads.
+    e->block = dim3(256, 1, 1);
     if (block_env) parse_block(block_env, e->block);
     if (e->pop.reserve(popb) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate %.2f GB of populations", popb / 1e9));

@@ -277,18 +277,21 @@ def test_cpp_lbmsolver_program_reproduces_reference(gpu, scene, tmp_path):
 
 @pytest.mark.parametrize("precision", ["f64", "f16s"])
 def test_lane_shift_path_equals_plain_loads(gpu, precision, monkeypatch):
-    """The odd sweep's aligned-load + lane-shift path (full 64-cell rows) gives the same
-    bits as plain shifted loads."""
+    """The odd sweep's aligned-load + lane-shift path (64x4 tiles of full 64-cell rows)
+    gives the same bits as plain shifted loads, on 64x4 tiles and on the default 256x1
+    row segments."""
     cfg, solid, src, probes = scenes.box_scene(precision, nx=128, ny=24, nz=16)
     cfg.stream_pattern = nlos.PATTERN_INPLACE
     runs = []
-    for flag in ("1", "0"):
+    for block, flag in (("64x4", "1"), ("64x4", "0"), ("256x1", "1")):
+        monkeypatch.setenv("NLOS_BLOCK", block)
         monkeypatch.setenv("NLOS_DPP", flag)
         e = _engine_run(cfg, solid, src, probes, 21)
         runs.append((e.read_probes(), e.debug_read_f()))
         e.close()
-    assert np.array_equal(runs[0][0], runs[1][0])
-    assert np.array_equal(runs[0][1], runs[1][1], equal_nan=True)
+    for r in runs[1:]:
+        assert np.array_equal(runs[0][0], r[0])
+        assert np.array_equal(runs[0][1], r[1], equal_nan=True)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32", "f16s"])
