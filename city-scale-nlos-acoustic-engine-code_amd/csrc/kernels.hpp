@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -619,6 +621,9 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
     using S = typename Tr<MODE>::S;
     const long long N = a.g.ncell;
     const long long c = cb + t;
+    // lattice slot s0 + r holds slot r; the B16 mantissa arrays hold slots 1..18
+    constexpr int S0 = MODE == M_B16 ? 1 : 0;
+    const std::conditional_t<MODE == M_B16, Lanes<M_B16, Q - 1>, Lanes<MODE>> L(cb, t, N);
     const uint8_t m = in.m;
     C f[Q];
     if (m & MK_SOLID) {
@@ -639,8 +644,8 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
                 // B16: slot INV[q] of the cell itself, in its own exponent (w_INV[q] = w_q)
-                if constexpr (MODE == M_B16) bb[q] = b16_dec(cur[Lanes<M_B16, Q - 1>(cb, t, N).self(INV[q] - 1)], in.e[0]);
-                else bb[q] = static_cast<C>(cur[Lanes<MODE>(cb, t, N).self(INV[q])]);
+                if constexpr (MODE == M_B16) bb[q] = b16_dec(cur[L.self(INV[q] - S0)], in.e[0]);
+                else bb[q] = static_cast<C>(cur[L.self(INV[q])]);
             }
 #pragma unroll
             for (int q = 1; q < Q; ++q) {
@@ -716,11 +721,11 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
         __builtin_nontemporal_store(f[0], glb(b16_rest(a.pop_out)) + cb + t);
         const auto nm = glb(b16_mants(a.pop_out, N));
 #pragma unroll
-        for (int q = 1; q < Q; ++q) __builtin_nontemporal_store(bm[q], nm + Lanes<M_B16, Q - 1>(cb, t, N).self(q - 1));
+        for (int q = 1; q < Q; ++q) __builtin_nontemporal_store(bm[q], nm + L.self(q - S0));
         (glb(b16_exps(a.pop_out, N)) + cb)[t] = (signed char)(be - B16_BITS);
     } else {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(static_cast<S>(f[q]), nxt + Lanes<MODE>(cb, t, N).self(q));
+        for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(static_cast<S>(f[q]), nxt + L.self(q));
     }
 }
 
@@ -744,6 +749,8 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     const long long cb = cell_index(a.g, i0, j0, k);
     const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;
     const long long c = cb + t;
+    constexpr int S0 = MODE == M_B16 ? 1 : 0;   // see ab_cell
+    const std::conditional_t<MODE == M_B16, Lanes<M_B16, Q - 1>, Lanes<MODE>> L(cb, t, N);
     AbIn<MODE> in;
     // x-halo sources (see StepArgs::xh): slot q of (i - EX[q], j', k') with i - EX[q] = 0 or
     // nx + 1 comes from the record of row (j', k') instead of the lattice
@@ -763,7 +770,7 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
             if (xsrc(q)) in.v[q] = xm[xh_index(a.g, q, j - EY[q], k - EZ[q])];
-            else in.v[q] = mn[Lanes<M_B16, Q - 1>(cb, t, N).from(a.g, q, q - 1)];
+            else in.v[q] = mn[L.from(a.g, q, q - S0)];
         }
         in.v[0] = 0;
     } else {
@@ -773,7 +780,7 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
         for (int q = 0; q < Q; ++q) {
             S v;
             if (xsrc(q)) v = xs[xh_index(a.g, q, j - EY[q], k - EZ[q])];
-            else v = cur[Lanes<MODE>(cb, t, N).from(a.g, q, q)];
+            else v = cur[L.from(a.g, q, q)];
             if constexpr (MODE == M_F16) {
                 if (q & 1) in.v[q >> 1].y = v;
                 else in.v[q >> 1].x = v;

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import json
 import os
+import struct
 
 import numpy as np
 import pytest
@@ -168,6 +169,20 @@ def test_checkpoint_restore_is_bitwise(gpu, precision, pattern):
     r.synchronize()
     assert np.array_equal(r.read_probes(), p1)
     assert np.array_equal(r.debug_read_f(), f1, equal_nan=True)
+
+
+def test_checkpoint_of_another_format_version_is_refused(gpu):
+    """Format version 3 holds the 16-bit modes' populations blocked by 64 cells; a blob that
+    claims another version (bytes 8..11 after the magic) is refused, whatever its size."""
+    cfg, solid, src, probes = scenes.box_scene("f16s")
+    e = _engine_run(cfg, solid, src, probes, 2)
+    blob = bytearray(e.checkpoint())
+    assert struct.unpack_from("<i", blob, 8)[0] == 3
+    struct.pack_into("<i", blob, 8, 2)
+    with pytest.raises(nlos.EngineError) as ex:
+        e.restore(bytes(blob))
+    assert ex.value.code == 1 and "version 2" in str(ex.value)
+    e.close()
 
 
 def test_nonfinite_population_is_reported(gpu):
