@@ -648,6 +648,16 @@ def main() -> None:
     for p in ("b16", "f16s", "f32"):
         if "f64" in res and p in res:
             out[f"speedup_{p}_vs_f64"] = round(res[p]["mlups"] / res["f64"]["mlups"], 3)
+    if "f16s" in res and res["f16s"]["pattern"] == PATTERN_NAMES[nlos.PATTERN_TWO_ARRAY]:
+        # The FP16S sweep's loads and stores alone (no mask, bounce-back or collision) on the
+        # same grid: what the access pattern allows, at 76 B per cell.
+        pms = nlos.pull_pattern_ms(args.nx, args.ny, args.nz, 5)
+        moved = args.nx * args.ny * args.nz * 76
+        out["pattern_ceiling"] = {"avg_ms": round(pms, 4), "bytes_per_cell": 76,
+                                  "gbs": round(moved / (pms * 1e-3) / 1e9, 1),
+                                  "frac": round(moved / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                  "f16s_sweep_over_pattern": round(res["f16s"]["kernel_avg_ms"] / pms, 4)}
+        log(f"pull pattern alone: {pms:.4f} ms ({out['pattern_ceiling']['frac']:.3f} of peak)")
     if args.error_steps > 0 and "f64" in res:
         # Probe error of the reduced-precision paths against FP64 over the same steps
         # (relative L2 over all 64 probe traces, and the worst single probe).

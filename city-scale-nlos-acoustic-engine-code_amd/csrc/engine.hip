@@ -846,6 +846,19 @@ int ready(const nlos_engine *e) {
     return NLOS_OK;
 }
 
+// Grid of an nx*ny*nz interior (see GridDesc): rows padded to 64 cells, interior rows
+// starting 64-aligned at i = 1, ncell a multiple of 64.
+GridDesc make_grid(int nx, int ny, int nz) {
+    GridDesc g{};
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.px = ((nx + 2 + 63) / 64) * 64;
+    g.py = ny + 2; g.pz = nz + 2;
+    g.plane = (long long)g.px * g.py;
+    g.origin = 63;
+    g.ncell = ((g.origin + g.plane * g.pz + 63) / 64) * 64;
+    return g;
+}
+
 int parse_block(const char *s, dim3 &b) {
     int x = 0, y = 0;
     if (!s || std::sscanf(s, "%dx%d", &x, &y) != 2) return 0;
@@ -927,12 +940,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         e->cfg.nscbc_faces = 0;
         e->cfg.stream_pattern = NLOS_PATTERN_TWO_ARRAY;
     }
-    e->g.nx = NX; e->g.ny = NY; e->g.nz = NZ;
-    e->g.px = ((NX + 2 + 63) / 64) * 64;
-    e->g.py = NY + 2; e->g.pz = NZ + 2;
-    e->g.plane = (long long)e->g.px * e->g.py;
-    e->g.origin = 63;
-    e->g.ncell = ((e->g.origin + e->g.plane * e->g.pz + 63) / 64) * 64;
+    e->g = make_grid(NX, NY, NZ);
     e->dt = (CS * cfg->dx) / cfg->c_phys;     // 05:200-203
     if (fam_a) e->dt = std::sqrt(1.0 / 3.0) * cfg->dx / cfg->c_phys;     // 01:37-41
     e->omega = 1.0 / cfg->tau;                // 05:908
@@ -1758,30 +1766,54 @@ NLOS_API int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, in
     return NLOS_OK;
 }
 
+namespace {
+// Times `launch` repeated `reps` times on a private stream after one untimed launch; the
+// stream and events are released on every return path.
+template <class F>
+int time_launches(int reps, double *avg_ms, F launch) {
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int rc) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        return done(fail(NLOS_ERR_HIP, "cannot create a stream or events for timing"));
+    hipError_t r = launch(s);
+    if (r == hipSuccess) r = hipEventRecord(e0, s);
+    for (int k = 0; k < reps && r == hipSuccess; ++k) r = launch(s);
+    if (r == hipSuccess) r = hipEventRecord(e1, s);
+    if (r == hipSuccess) r = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
+    if (r != hipSuccess) return done(fail(NLOS_ERR_HIP, "timed launches failed: %s", hipGetErrorString(r)));
+    *avg_ms = ms / reps;
+    return done(NLOS_OK);
+}
+}  // namespace
+
+NLOS_API int nlos_debug_pull_pattern(int32_t nx, int32_t ny, int32_t nz, int32_t reps, double *avg_ms) {
+    if (nx < 1 || ny < 1 || nz < 1 || reps < 1 || !avg_ms) return fail(NLOS_ERR_ARG, "bad pattern request");
+    const GridDesc g = make_grid(nx, ny, nz);
+    const size_t bytes = (size_t)g.ncell * Q * 2;
+    ScopedBuf a, b;
+    if (a.reserve(bytes) != hipSuccess || b.reserve(bytes) != hipSuccess)
+        return fail(NLOS_ERR_NOMEM, "cannot allocate two %.2f GB pattern arrays", bytes / 1e9);
+    HIP_TRY(hipMemset(a.p, 0, bytes));
+    return time_launches(reps, avg_ms, [&](hipStream_t s) { return launch_pull_pattern16(a.p, b.p, g, s); });
+}
+
 NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) {
     if (bytes < 16 || reps < 1 || !avg_ms) return fail(NLOS_ERR_ARG, "bad copy request");
     const long long n16 = (long long)(bytes / 16);
     ScopedBuf a, b;
     HIP_TRY(a.reserve((size_t)n16 * 16));
     HIP_TRY(b.reserve((size_t)n16 * 16));
-    hipStream_t s;
-    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipMemsetAsync(a.p, 0, (size_t)n16 * 16, s));
-    HIP_TRY(launch_copy16(a.p, b.p, n16, s));
-    HIP_TRY(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r) HIP_TRY(launch_copy16(a.p, b.p, n16, s));
-    HIP_TRY(hipEventRecord(e1, s));
-    HIP_TRY(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    *avg_ms = ms / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
-    return NLOS_OK;
+    HIP_TRY(hipMemset(a.p, 0, (size_t)n16 * 16));
+    return time_launches(reps, avg_ms, [&](hipStream_t s) { return launch_copy16(a.p, b.p, n16, s); });
 }
 
 // =========================================================================================

@@ -68,6 +68,9 @@ hipError_t launch_wall_bits(const uint8_t *mask, const GridDesc &g, uint32_t *bi
 hipError_t launch_count(long long *ctr, long long by, hipStream_t s);
 // Grid-stride copy of n16 16-byte words.
 hipError_t launch_copy16(const void *a, void *b, long long n16, hipStream_t s);
+// The FP16 two-array sweep's loads and stores alone (kernels_f16.hip); in and out are blocked
+// FP16 population arrays of g.ncell cells.
+hipError_t launch_pull_pattern16(const void *in, void *out, const GridDesc &g, hipStream_t s);
 
 // Legacy family-A step (legacy_a.hip): the halo of 01's exchange_halos, then collide and
 // stream (01's step_pull).

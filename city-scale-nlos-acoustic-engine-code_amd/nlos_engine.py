@@ -135,6 +135,7 @@ SIGNATURES = {
     "nlos_debug_read_f": (C.c_int, [_E, _P]),
     "nlos_debug_poke": (C.c_int, [_E, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
     "nlos_debug_copy_kernel": (C.c_int, [C.c_size_t, C.c_int32, C.POINTER(C.c_double)]),
+    "nlos_debug_pull_pattern": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -436,6 +437,14 @@ def copy_kernel_ms(nbytes: int, reps: int = 5) -> float:
     """Mean time of a plain device copy of nbytes (read nbytes, write nbytes)."""
     ms = C.c_double()
     _check(load_library().nlos_debug_copy_kernel(nbytes, reps, C.byref(ms)))
+    return ms.value
+
+
+def pull_pattern_ms(nx: int, ny: int, nz: int, reps: int = 5) -> float:
+    """Mean time of the FP16S two-array sweep's loads and stores alone on an nx*ny*nz grid
+    (19 blocked FP16 slots pulled and stored per interior cell; no arithmetic)."""
+    ms = C.c_double()
+    _check(load_library().nlos_debug_pull_pattern(nx, ny, nz, reps, C.byref(ms)))
     return ms.value
 
 

@@ -25,6 +25,13 @@ int nlos_debug_poke(nlos_engine *e, int32_t i, int32_t j, int32_t k, int32_t q, 
  * from HIP events.  Used to calibrate HBM byte counters and as the practical copy rate. */
 int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms);
 
+/* The data movement of the FP16S two-array sweep alone on an nx*ny*nz grid: every interior
+ * cell loads the 19 blocked FP16 slots it pulls and stores them into its own slots of a
+ * second array (non-temporal), with no mask, bounce-back or collision.  Repeated `reps`
+ * times on a private stream; *avg_ms is the mean launch time from HIP events.  The
+ * practical ceiling of that access pattern, to set the sweep's time against. */
+int nlos_debug_pull_pattern(int32_t nx, int32_t ny, int32_t nz, int32_t reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
