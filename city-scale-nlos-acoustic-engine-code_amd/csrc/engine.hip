@@ -1807,7 +1807,8 @@ NLOS_API int nlos_debug_pull_pattern(int32_t nx, int32_t ny, int32_t nz, int32_t
 }
 
 NLOS_API int nlos_debug_copy_kernel(size_t bytes, int32_t reps, double *avg_ms) {
-    if (bytes < 16 || reps < 1 || !avg_ms) return fail(NLOS_ERR_ARG, "bad copy request");
+    // one thread per 16 bytes: the grid stays below 2^31 workgroups up to 2^43 bytes
+    if (bytes < 16 || bytes > ((size_t)1 << 40) || reps < 1 || !avg_ms) return fail(NLOS_ERR_ARG, "bad copy request");
     const long long n16 = (long long)(bytes / 16);
     ScopedBuf a, b;
     HIP_TRY(a.reserve((size_t)n16 * 16));
