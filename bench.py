@@ -664,8 +664,10 @@ def main() -> None:
         log(f"probe error run: {args.error_steps} steps per precision")
         traces = {}
         for p in res:
+            t = time.perf_counter()
             traces[p] = run_precision(wl, stl, p, 0, args.error_steps, timing=False,
                                       pattern=args.pattern)["probes"]
+            log(f"  {p}: {args.error_steps} steps in {time.perf_counter() - t:.1f} s")
         ref = traces["f64"]
         err = {"steps": args.error_steps, "signal_max": float(np.abs(ref).max())}
         for p in traces:
