@@ -1011,7 +1011,12 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
     }
     const long long N = g.ncell;
     const long long h = cell_index(g, i, j, k);
-    if (a.mask[h] & MK_SOLID) return;
+    // The halo cell's mask, the interior neighbour's mask and the interior neighbour's
+    // populations are loaded together (their addresses need no memory); a solid halo cell
+    // returns, a solid neighbour or (LAY_Q) a neighbour next to a wall then overrides the
+    // populations.  One memory round trip instead of three: at 1024x1024x256 the kernel
+    // takes 43.9 instead of 45.3 us (FP16S) and 93.6 instead of 95.7 us (FP64).
+    const uint8_t mh = a.mask[h];
 
     const int oi = i == 0 ? -1 : (i == nx + 1 ? 1 : 0);
     const int oj = j == 0 ? -1 : (j == ny + 1 ? 1 : 0);
@@ -1028,10 +1033,13 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
     // Boundary state of an NSCBC face cell (05:689-699 and its five variants).
     C fI[Q];
     C rI = 1, uI[3] = {0, 0, 0}, rB = 1, uB[3] = {0, 0, 0}, dI = 0, dB = 0, u2I = 0, u2B = 0;
+    const int ii = i - oi, jj = j - oj, kk = k - ok;
+    const long long I = cell_index(g, ii, jj, kk);
+    uint8_t mI = 0;
     if (ns) {
-        const int ii = i - oi, jj = j - oj, kk = k - ok;
-        const long long I = cell_index(g, ii, jj, kk);
-        if (axis == 0 && a.fb[face & 1] && !(a.mask[I] & MK_SOLID)) {
+        mI = a.mask[I];
+        // the populations of a fluid neighbour, in every layout
+        if (axis == 0 && a.fb[face & 1]) {
             const long long r0 = ((long long)(jj - 1) + (long long)ny * (kk - 1)) * Q;
             if constexpr (MODE == M_B16) {
                 const float *fb = static_cast<const float *>(a.fb[face & 1]) + r0;
@@ -1042,7 +1050,21 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
 #pragma unroll
                 for (int q = 0; q < Q; ++q) fI[q] = ld<MODE>(fb, q, a.inv_scale);   // per-thread address
             }
-        } else if (a.mask[I] & MK_SOLID) {
+        } else if constexpr (MODE == M_B16) {
+            b16_load_g(a.pop, g.ncell, I, fI);
+        } else {
+            // f*_q(I) in slot q of I (LAY_AB), slot INV[q] of I (LAY_P), or, with no solid
+            // neighbour, slot q of I + e_q (LAY_Q)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                fI[q] = ld<MODE>(a.pop, LAY == LAY_AB ? pop_index<MODE>(q, I, N)
+                                        : LAY == LAY_P ? pop_index<MODE>(INV[q], I, N)
+                                                       : pop_index<MODE>(q, I + nb_off(g, q), N), a.inv_scale);
+        }
+    }
+    if (mh & MK_SOLID) return;
+    if (ns) {
+        if (mI & MK_SOLID) {
             int pidx;
             if (axis == 0) pidx = (jj - 1) + ny * (kk - 1);
             else if (axis == 1) pidx = (ii - 1) + nx * (kk - 1);
@@ -1051,19 +1073,11 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
             const C *so = static_cast<const C *>(a.s_old);
 #pragma unroll
             for (int q = 0; q < Q; ++q) fI[q] = so[(long long)slot * Q + q];
-        } else if constexpr (MODE == M_B16) {
-            b16_load_g(a.pop, g.ncell, I, fI);
-        } else if (LAY != LAY_Q || !(a.mask[I] & MK_NEAR)) {
-            // f*_q(I) in slot q of I (LAY_AB), slot INV[q] of I (LAY_P), or, with no solid
-            // neighbour, slot q of I + e_q (LAY_Q)
+        } else if constexpr (MODE != M_B16 && LAY == LAY_Q) {
+            if (mI & MK_NEAR) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                fI[q] = ld<MODE>(a.pop, LAY == LAY_AB ? pop_index<MODE>(q, I, N)
-                                        : LAY == LAY_P ? pop_index<MODE>(INV[q], I, N)
-                                                       : pop_index<MODE>(q, I + nb_off(g, q), N), a.inv_scale);
-        } else {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) fI[q] = read_post<MODE, LAY>(a.pop, a.mask, g, I, q, a.inv_scale);
+                for (int q = 0; q < Q; ++q) fI[q] = read_post<MODE, LAY>(a.pop, a.mask, g, I, q, a.inv_scale);
+            }
         }
         const bool plus = (oi + oj + ok) > 0;
         if constexpr (MODE == M_F64) {
