@@ -171,6 +171,48 @@ def test_checkpoint_restore_is_bitwise(gpu, precision, pattern):
     assert np.array_equal(r.debug_read_f(), f1, equal_nan=True)
 
 
+@pytest.mark.parametrize("precision,scene", [("f32", "box"), ("f32", "wide"), ("f32", "reflect"),
+                                             ("f16s", "box"), ("f16s", "wide")])
+def test_inplace_equals_two_array_bitwise(gpu, precision, scene):
+    """The in-place A-A sweep and the two-array pull sweep are two kernels and two layouts
+    of the same step: probes, field and populations agree bit for bit (rows of 40, 100 and
+    30 cells, so no row is a whole number of 64-cell blocks; partial reflection in FP32).
+    Scaled FP16 with partial reflection is not bitwise: the in-place even sweep stores the
+    reflected wall copy (rounding after reflecting), the pull sweep reflects the stored,
+    rounded value; test_inplace_and_two_array_fp16_reflection_agree_to_rounding covers it."""
+    if scene == "reflect":
+        cfg, solid, src, probes = scenes.reflect_scene(precision)
+    else:
+        dims = {} if scene == "box" else {"nx": 100, "ny": 10, "nz": 37}
+        cfg, solid, src, probes = scenes.box_scene(precision, **dims)
+    runs = []
+    for pattern in PATTERNS:
+        cfg.stream_pattern = nlos.PATTERNS[pattern]
+        e = _engine_run(cfg, solid, src, probes, 33)
+        runs.append((e.read_probes(), e.read_field(), e.debug_read_f()[solid == 0]))
+        e.close()
+    assert np.abs(runs[0][1]).max() > 0
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
+def test_inplace_and_two_array_fp16_reflection_agree_to_rounding(gpu):
+    """Scaled FP16 with wall_reflect = 0.8: the two patterns round the reflected wall values
+    in a different order, so they agree to FP16 rounding noise, far inside the FP16S
+    tolerance against FP64 (TOL_F16)."""
+    cfg, solid, src, probes = scenes.reflect_scene("f16s")
+    runs = []
+    for pattern in PATTERNS:
+        cfg.stream_pattern = nlos.PATTERNS[pattern]
+        e = _engine_run(cfg, solid, src, probes, 33)
+        runs.append((e.read_probes(), e.read_field()))
+        e.close()
+    perr = scenes.rel_l2(runs[0][0], runs[1][0])
+    ferr = scenes.rel_l2(runs[0][1], runs[1][1])
+    print(f"  in place vs two arrays: probe {perr:.2e}, field {ferr:.2e}")
+    assert 0.0 < ferr and perr <= 1e-2 * TOL_F16 and ferr <= 1e-2 * TOL_F16
+
+
 def test_checkpoint_of_another_format_version_is_refused(gpu):
     """Format version 3 holds the 16-bit modes' populations blocked by 64 cells; a blob that
     claims another version (bytes 8..11 after the magic) is refused, whatever its size."""
