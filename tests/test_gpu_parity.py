@@ -113,9 +113,12 @@ def test_reduced_precision_vs_f64(gpu, precision, tol, scene, pattern):
     e.close()
 
 
-@pytest.mark.parametrize("pattern", PATTERNS)
-def test_three_phase_api_equals_fused_step(gpu, pattern):
-    cfg, solid, src, probes = scenes.box_scene("f64")
+@pytest.mark.parametrize("precision,pattern", [("f64", "inplace"), ("f64", "two_array"),
+                                               ("f16s", "two_array"), ("b16", "two_array")])
+def test_three_phase_api_equals_fused_step(gpu, precision, pattern):
+    """apply_source, exchange_halos and step_collision driven one by one equal nlos_step
+    (the x-halo records of the 16-bit two-array runs included)."""
+    cfg, solid, src, probes = scenes.box_scene(precision)
     cfg.stream_pattern = nlos.PATTERNS[pattern]
     a = _engine_run(cfg, solid, src, probes, 25)
     b = nlos.Engine(cfg)
@@ -211,6 +214,26 @@ def test_inplace_and_two_array_fp16_reflection_agree_to_rounding(gpu):
     ferr = scenes.rel_l2(runs[0][1], runs[1][1])
     print(f"  in place vs two arrays: probe {perr:.2e}, field {ferr:.2e}")
     assert 0.0 < ferr and perr <= 1e-2 * TOL_F16 and ferr <= 1e-2 * TOL_F16
+
+
+@pytest.mark.parametrize("precision", ["f64", "f16s", "b16"])
+def test_field_capture_leaves_the_state_unchanged(gpu, precision):
+    """The capture variants of the sweep write rho-1 besides the step: the same populations
+    and probes with the field captured on the last step or never."""
+    cfg, solid, src, probes = scenes.box_scene(precision)
+    cfg.stream_pattern = nlos.PATTERN_TWO_ARRAY
+    a = _engine_run(cfg, solid, src, probes, 19)            # captures the last step
+    b = nlos.Engine(cfg)
+    b.set_mask(solid)
+    b.set_source(src)
+    b.set_probes(probes)
+    b.step(19)
+    b.synchronize()
+    assert np.array_equal(a.read_probes(), b.read_probes())
+    assert np.array_equal(a.debug_read_f(), b.debug_read_f(), equal_nan=True)
+    assert np.abs(a.read_field()).max() > 0
+    a.close()
+    b.close()
 
 
 def test_checkpoint_of_another_format_version_is_refused(gpu):
