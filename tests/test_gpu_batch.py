@@ -97,3 +97,12 @@ def test_batch_scene_cannot_step_alone(gpu):
     b.step(2)
     assert b.scene(1).info().step == 2
     b.close()
+
+
+def test_batch_of_mixed_precisions_is_refused(gpu):
+    """A batch's scenes share one storage precision (one kernel set per launch)."""
+    a = _defs("f32")[0][0]
+    b = _defs("f16s")[0][0]
+    with pytest.raises(nlos.EngineError) as ex:
+        nlos.Batch([a, b])
+    assert ex.value.code == 1
