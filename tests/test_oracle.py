@@ -254,3 +254,147 @@ def test_missing_stl_keeps_empty_geometry():
     o = Oracle(_small_cfg())
     assert o.load_stl(b"short") == 1
     assert o.get_mask().sum() == 0
+
+
+# ---- branches the compiled reference does not run: wall reflection < 1, the z- face as an
+# open boundary (NSCBC and sponge) and an open (fluid) ground halo.  C1-, C4- and C5-shaped
+# workloads use them, so each is pinned by a closed form or an exact symmetry that ties it
+# to golden-pinned code.
+
+def _feq_np(rho, u):
+    """Textbook D3Q19 equilibrium w (rho + 3 e.j + 4.5 (e.j)^2/rho - 1.5 j^2/rho), j = rho u."""
+    e = np.stack([EX, EY, EZ], axis=1).astype(np.float64)
+    cu = e @ np.asarray(u, np.float64)
+    return W * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * float(np.dot(u, u)))
+
+
+def _reflect_scene(alpha, ground=1):
+    n = (14, 12, 10)
+    cfg = _small_cfg(x_max=float(n[0]), y_max=float(n[1]), z_max=float(n[2]), sponge_faces=0,
+                     nscbc_faces=0, wall_reflect=alpha, ground=ground)
+    m = np.zeros(n[::-1], np.uint8)
+    m[:4, 3:7, 4:8] = 1            # a block on the ground
+    m[2:6, 8:10, 10:12] = 1        # a floating block (walls above and below fluid)
+    m[7, 1:3, 1:3] = 1
+    return cfg, m
+
+
+def _pull_sources(m, ground):
+    """Per interior cell and direction: 0 fluid interior source, 1 solid source (bounce),
+    2 source on the open halo (holds the rest state w)."""
+    nz, ny, nx = m.shape
+    pad = np.full((nz + 2, ny + 2, nx + 2), 2, np.int8)     # halo: rest state
+    pad[1:-1, 1:-1, 1:-1] = m
+    if ground:
+        pad[0] = 1                                          # ground halo is solid
+    kinds = np.empty((nz, ny, nx, 19), np.int8)
+    for q in range(19):
+        kinds[..., q] = pad[1 - EZ[q]:1 - EZ[q] + nz, 1 - EY[q]:1 - EY[q] + ny, 1 - EX[q]:1 - EX[q] + nx]
+    return kinds
+
+
+@pytest.mark.parametrize("alpha", [1.0, 0.8, 0.35])
+def test_wall_reflection_closed_form_uniform_state(alpha):
+    """From a uniform state f = w (1 + delta), one step pulls w (1 + delta) across fluid
+    links, w from the open halo, and w (1 + alpha delta) off a wall (the bounced deviation
+    scaled by alpha, the rest part kept).  So every fluid cell's density after one step is
+    1 + delta (1 - sum_halo w_q - (1 - alpha) sum_wall w_q).  Reflecting f instead of f - w,
+    or scaling by 1 - alpha, fails this."""
+    delta = 1e-3
+    cfg, m = _reflect_scene(alpha)
+    o = Oracle(cfg)
+    o.set_mask(m)
+    o.set_density(np.full(m.shape, 1.0 + delta))
+    o.run(1)
+    kinds = _pull_sources(m, ground=1)
+    want = 1.0 + delta * (1.0 - (W * (kinds == 2)).sum(-1) - (1.0 - alpha) * (W * (kinds == 1)).sum(-1))
+    got = o.read_rho()
+    fluid = m == 0
+    assert np.any(fluid & (kinds == 1).any(-1))             # walls are hit
+    np.testing.assert_allclose(got[fluid], want[fluid], atol=1e-15, rtol=0)
+
+
+@pytest.mark.parametrize("alpha", [0.8, 0.35])
+def test_wall_reflection_pulls_the_inverse_population(alpha):
+    """With a uniform moving state f = feq(rho, u) the populations of a cell differ by
+    direction.  A wall returns population inv(q) of the cell itself, reflected about rest:
+    f_in_q = w_q + alpha (feq_inv(q) - w_q).  The first moments of the pulled set then have
+    a closed form per cell; pulling f_q instead of f_inv(q) changes them."""
+    cfg, m = _reflect_scene(alpha)
+    o = Oracle(cfg)
+    o.set_mask(m)
+    rho0, u = 1.0 + 2e-3, np.array([3e-3, -2e-3, 1.5e-3])
+    fe = _feq_np(rho0, u)
+    f = o.read_f()
+    f[1:-1, 1:-1, 1:-1][m == 0] = fe
+    o.write_f(f)
+    o.run(1)
+    kinds = _pull_sources(m, ground=1)
+    inv = np.array([0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15])
+    pulled = np.where(kinds == 0, fe, np.where(kinds == 2, W, W + alpha * (fe[inv] - W)))
+    want = pulled.sum(-1)
+    fluid = m == 0
+    np.testing.assert_allclose(o.read_rho()[fluid], want[fluid], atol=2e-15, rtol=0)
+    # the bounce-back inverse matters: with f_q instead of f_inv(q) some cells differ
+    wrong = np.where(kinds == 0, fe, np.where(kinds == 2, W, W + alpha * (fe - W))).sum(-1)
+    assert np.abs(wrong - want)[fluid].max() > 1e-6
+
+
+def test_sponge_values_on_the_bottom_face():
+    """The z- sponge uses the distance of the cell centre to z_min, like the other faces."""
+    cfg = _small_cfg(x_max=30.0, y_max=30.0, z_max=20.0, sponge_width=6.0, sponge_strength=0.3,
+                     sponge_faces=nlos.FACE_ZM | nlos.FACE_ZP, ground=0)
+    s = Oracle(cfg).sponge()
+    for k in range(6):
+        d = k + 0.5
+        want = 0.3 * ((6.0 - d) / 6.0) * ((6.0 - d) / 6.0)
+        assert s[k, 15, 15] == want, k
+        assert s[19 - k, 15, 15] == want, k
+    assert np.all(s[6:14] == 0.0)
+    cfg2 = _small_cfg(x_max=30.0, y_max=30.0, z_max=20.0, sponge_width=6.0, sponge_strength=0.3,
+                      sponge_faces=nlos.FACE_ZM | nlos.FACE_XM, ground=0)
+    s2 = Oracle(cfg2).sponge()
+    # the nearer face wins: at (i=2, k=4) x- is 2.5 away, z- 4.5
+    assert s2[4, 15, 2] == 0.3 * ((6.0 - 2.5) / 6.0) ** 2
+    assert s2[2, 15, 20] == 0.3 * ((6.0 - 2.5) / 6.0) ** 2
+
+
+@pytest.mark.parametrize("faces", ["zonly", "all"])
+def test_open_bottom_mirrors_open_top(faces):
+    """A scene symmetric under z -> z_max - z, with an open (fluid) ground halo and NSCBC
+    plus sponge on both z faces, stays symmetric: the z- ghost (Wp = 0, 05:684-704's rule
+    on the bottom face) must mirror the golden-pinned z+ ghost (05:804-835), and the z-
+    sponge the z+ one.  Swapping Wp/Wm on z-, dropping the z- sponge or closing the ground
+    breaks the symmetry by far more than rounding."""
+    n = 16
+    zf = nlos.FACE_ZM | nlos.FACE_ZP
+    fs = zf if faces == "zonly" else nlos.FACES_ALL
+    cfg = _small_cfg(x_max=float(n), y_max=float(n), z_max=float(n), sponge_width=3.0,
+                     sponge_strength=0.1, sponge_faces=fs, nscbc_faces=fs, ground=0, tau=0.56)
+    m = np.zeros((n, n, n), np.uint8)
+    m[6:10, 5:9, 3:7] = 1          # symmetric about the z mid-plane
+    o = Oracle(cfg)
+    o.set_mask(m)
+    z, y, x = np.meshgrid(*(np.arange(n) + 0.5,) * 3, indexing="ij")
+    rho = 1.0 + 1e-3 * np.exp(-((x - 10.3) ** 2 + (y - 7.1) ** 2 + (z - 8.0) ** 2) / 4.0)
+    o.set_density(rho)
+    o.run(30)
+    r = o.read_rho() - 1.0
+    # the wave has crossed both z faces: the open boundary and the sponge acted
+    assert np.abs(r[0]).max() > 1e-6 and np.abs(r[-1]).max() > 1e-6
+    # mirrored populations are summed in another order: rounding differences of ~1e-15
+    np.testing.assert_allclose(r, r[::-1], atol=1e-13, rtol=0)
+    f = o.read_f()[1:-1, 1:-1, 1:-1]
+    mirror = [int(np.flatnonzero((EX == EX[q]) & (EY == EY[q]) & (EZ == -EZ[q]))[0]) for q in range(19)]
+    np.testing.assert_allclose(f, f[::-1][..., mirror], atol=1e-13, rtol=0)
+
+
+def test_rest_state_is_fixed_with_open_ground_and_six_open_faces():
+    o = Oracle(_small_cfg(sponge_faces=nlos.FACES_ALL, nscbc_faces=nlos.FACES_ALL, ground=0,
+                          wall_reflect=0.7))
+    m = np.zeros((12, 14, 16), np.uint8)
+    m[3:8, 3:7, 4:9] = 1
+    o.set_mask(m)
+    o.run(20)
+    f = o.read_f()[1:-1, 1:-1, 1:-1][m == 0]
+    np.testing.assert_allclose(f, np.broadcast_to(W, f.shape), rtol=1e-14)
