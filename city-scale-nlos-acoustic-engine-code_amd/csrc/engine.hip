@@ -841,8 +841,16 @@ int collect_timing(nlos_engine *e) {
     return NLOS_OK;
 }
 
+int batch_settle(const nlos_batch *b);
+
+// Every entry point that takes an engine starts here.  A scene of a batch is advanced on
+// the batch's stream, and nlos_batch_step returns with that work still in flight; the
+// scene's own calls (reads, checkpoints, VTK, dumps, debug access) go through the scene's
+// stream or synchronous copies, which do not wait for the batch's non-blocking stream.  So
+// for a batch scene the batch's stream is drained first.
 int ready(const nlos_engine *e) {
     if (!e) return fail(NLOS_ERR_ARG, "null engine");
+    if (e->batch) return batch_settle(e->batch);
     return NLOS_OK;
 }
 
@@ -1854,6 +1862,12 @@ int batch_ready(const nlos_batch *b) {
     if (!b) return fail(NLOS_ERR_ARG, "null batch");
     return NLOS_OK;
 }
+
+int batch_settle(const nlos_batch *b) {
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return NLOS_OK;
+}
+
 
 long long stencil_bound(const nlos_engine *e) {
     if (!e->has_source) return 0;

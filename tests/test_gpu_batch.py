@@ -106,3 +106,25 @@ def test_batch_of_mixed_precisions_is_refused(gpu):
     with pytest.raises(nlos.EngineError) as ex:
         nlos.Batch([a, b])
     assert ex.value.code == 1
+
+
+def test_scene_reads_wait_for_the_batch(gpu):
+    """nlos_batch_step returns with its graph replays still running on the batch's stream.
+    A scene read issued right after it (no nlos_batch_synchronize) must see every step:
+    probes, the field and the state equal the scenes stepped alone, bit for bit."""
+    defs = []
+    for seed in (11, 12):
+        cfg, solid, src, probes = scenes.box_scene("f32", nx=160, ny=128, nz=64, seed=seed)
+        cfg.stream_pattern = nlos.PATTERN_TWO_ARRAY
+        defs.append((cfg, solid, src, probes, None))
+    b = nlos.Batch([d[0] for d in defs])
+    for i, d in enumerate(defs):
+        _setup(b.scene(i), d)
+    b.step(300)
+    got = [(b.scene(i).read_probes(), b.scene(i).debug_read_f()) for i in range(len(defs))]
+    b.close()
+    for i, d in enumerate(defs):
+        want_p, want_f = _single(d, 300)
+        assert got[i][0].shape == want_p.shape == (len(d[3]), 300)
+        assert np.array_equal(got[i][0], want_p), i
+        assert np.array_equal(got[i][1], want_f), i
