@@ -20,8 +20,9 @@ from oracle_ffi import Oracle
 
 pytestmark = pytest.mark.gpu
 
-# Probe and field relative L2 against FP64 after 80 steps of the small scenes.
-TOL_B16 = 1e-2
+# Probe and field relative L2 against FP64 after 80 steps of the small scenes: the bound the
+# 16-bit storage meets at the benchmark configuration too.
+TOL_B16 = 1e-3
 W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
 
 

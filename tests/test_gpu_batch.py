@@ -127,4 +127,4 @@ def test_scene_reads_wait_for_the_batch(gpu):
         want_p, want_f = _single(d, 300)
         assert got[i][0].shape == want_p.shape == (len(d[3]), 300)
         assert np.array_equal(got[i][0], want_p), i
-        assert np.array_equal(got[i][1], want_f), i
+        assert np.array_equal(got[i][1], want_f, equal_nan=True), i
