@@ -13,6 +13,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -523,9 +524,11 @@ struct SourceDeriv {
     std::vector<double> dc;
 };
 
-// Appends the stencil cells and adds of time t (and their derivatives when d is set).
-void schedule_step(nlos_engine *e, double t, SourceDeriv *d = nullptr) {
-    if (!e->has_source) return;
+// The stencil cells and adds of time t into cells_out / adds_out (room for MAX_STENCIL
+// entries), and their derivatives when d is set.  Returns the number of entries.
+constexpr int MAX_STENCIL = 343;   // (2R+1)^3 at the largest stencil radius, 3
+int schedule_cells(nlos_engine *e, double t, long long *cells_out, double *adds_out, SourceDeriv *d = nullptr) {
+    if (!e->has_source) return 0;
     const nlos_config &c = e->cfg;
     const nlos_source &s = e->src;
     const double DX = c.dx;
@@ -566,7 +569,7 @@ void schedule_step(nlos_engine *e, double t, SourceDeriv *d = nullptr) {
         const int ic = (int)std::floor((cxs - c.x_min) / DX);
         const int jc = (int)std::floor((cys - c.y_min) / DX);
         const int kc = (int)std::floor((czs - c.z_min) / DX);
-        if (ic < -R || ic >= NX + R || jc < -R || jc >= NY + R || kc < -R || kc >= NZ + R) return;
+        if (ic < -R || ic >= NX + R || jc < -R || jc >= NY + R || kc < -R || kc >= NZ + R) return 0;
         for (int gk = kc - R; gk <= kc + R; ++gk)
             for (int gj = jc - R; gj <= jc + R; ++gj)
                 for (int gi = ic - R; gi <= ic + R; ++gi) {
@@ -588,11 +591,11 @@ void schedule_step(nlos_engine *e, double t, SourceDeriv *d = nullptr) {
                     ww[n++] = w;
                 }
     }
-    if (sumw < 1e-30) return;
+    if (sumw < 1e-30) return 0;
     const double A = (val * e->dt * s.gain) / sumw;
     for (int q = 0; q < n; ++q) {
-        e->sched_cells.push_back(cells[q]);
-        e->sched_adds.push_back(A * ww[q]);
+        cells_out[q] = cells[q];
+        adds_out[q] = A * ww[q];
     }
     if (d) {
         // add_q = val dt gain ww_q / S, S = sum ww: d/dc = val dt gain (gw_q S - ww_q sum gw) / S^2
@@ -605,6 +608,16 @@ void schedule_step(nlos_engine *e, double t, SourceDeriv *d = nullptr) {
             for (int a = 0; a < 3; ++a) d->dc.push_back(val * f * (gw[q][a] - ww[q] * sg[a] / sumw));
         }
     }
+    return n;
+}
+
+// Appends the stencil cells and adds of time t to the engine's host schedule.
+void schedule_step(nlos_engine *e, double t, SourceDeriv *d = nullptr) {
+    long long cells[MAX_STENCIL];
+    double adds[MAX_STENCIL];
+    const int n = schedule_cells(e, t, cells, adds, d);
+    e->sched_cells.insert(e->sched_cells.end(), cells, cells + n);
+    e->sched_adds.insert(e->sched_adds.end(), adds, adds + n);
 }
 
 int upload_schedule(nlos_engine *e) {
@@ -707,12 +720,15 @@ StepArgs step_args(const nlos_engine *e, int phase, dim3 block) {
     return a;
 }
 
-int launch_source(nlos_engine *e, long long off, long long n) {
+// Source adds of entries off .. off+n-1 of the schedule on the device: the engine's schedule
+// buffers, or the arrays given.
+int launch_source(nlos_engine *e, long long off, long long n, const long long *cells = nullptr,
+                  const double *adds = nullptr) {
     if (n <= 0) return NLOS_OK;
     if (int rc = sync_face_buffers(e)) return rc;
     SourceArgs a = source_args(e, 0);
-    a.cells = e->src_cells.as<long long>() + off;
-    a.adds = e->src_adds.as<double>() + off;
+    a.cells = (cells ? cells : e->src_cells.as<long long>()) + off;
+    a.adds = (adds ? adds : e->src_adds.as<double>()) + off;
     a.n = (int)n;
     HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::source(a, e->lay, e->stream); }));
     return NLOS_OK;
@@ -1309,6 +1325,8 @@ NLOS_API int nlos_set_probes(nlos_engine *e, const double *xyz, int32_t n, uint8
     if (!uc.empty()) {
         HIP_TRY(e->probe_cells.reserve(uc.size() * 8));
         HIP_TRY(hipMemcpy(e->probe_cells.p, uc.data(), uc.size() * 8, hipMemcpyHostToDevice));
+        // a first history, so that steps up to this count allocate nothing (nlos_step_noalloc)
+        if (int rc = ensure_hist(e, NLOS_PROBE_HISTORY_RESERVE)) return rc;
     }
     if (in_solid)
         for (int p = 0; p < n; ++p) in_solid[p] = ps[p].in_solid ? 1 : 0;
@@ -1382,6 +1400,96 @@ NLOS_API int nlos_step(nlos_engine *e, int64_t n, int32_t flags) {
             if ((rc = launch_sweep(e, cap))) return rc;
         }
         if (e->timing && (rc = collect_timing(e))) return rc;
+    }
+    return NLOS_OK;
+}
+
+// ---- the step without device allocation (nlos_step_noalloc) --------------------------------
+// The source schedule of nlos_step is staged in the engine's own device buffers, which grow
+// on demand.  nlos_step_noalloc stages it instead in `scratch`, a file-scope device array:
+// every GPU the library runs on holds its own copy until the process ends.  Host code never
+// passes the array's name as an address (that is a host-side stand-in); the address of each
+// device's copy comes from hipGetSymbolAddress, looked up once per device with that device
+// current, and kept in g_scratch.  Layout: NOALLOC_ENTRIES cell indices (int64), then
+// NOALLOC_ENTRIES adds (double).  Every byte read is written by the same call first, so
+// nothing needs clearing; calls that overlap on one GPU (two engines on two streams) race.
+constexpr long long NOALLOC_ENTRIES = NLOS_NOALLOC_SOURCE_ENTRIES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_ENTRIES * 16];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+constexpr long long NOALLOC_CHUNK = 4096;        // steps per staged chunk, at most
+std::mutex g_scratch_mu;                         // guards g_scratch and the host staging below
+unsigned char *g_scratch[MAX_DEVICES] = {};
+// host staging of one chunk (static storage: the path allocates no host memory either)
+long long g_sched_cells[NOALLOC_ENTRIES];
+double g_sched_adds[NOALLOC_ENTRIES];
+long long g_sched_off[NOALLOC_CHUNK + 1];
+
+int scratch_address(int device, unsigned char **out) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(NLOS_ERR_UNSUPPORTED, "device %d beyond %d", device, MAX_DEVICES);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(NLOS_ERR_HIP, "hipGetSymbolAddress returned no address for the staging array");
+        g_scratch[device] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch[device];
+    return NLOS_OK;
+}
+
+}  // namespace
+
+NLOS_API int nlos_step_noalloc(nlos_engine *e, int64_t n, int32_t flags) {
+    if (int rc = ready(e)) return rc;
+    if (e->batch) return fail(NLOS_ERR_STATE, "scene belongs to a batch; step it with nlos_batch_step");
+    if (n < 0) return fail(NLOS_ERR_ARG, "negative step count");
+    if (n == 0) return NLOS_OK;
+    if (e->family != NLOS_FAMILY_B) return fail(NLOS_ERR_UNSUPPORTED, "nlos_step_noalloc runs NLOS_FAMILY_B engines");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != e->device) return fail(NLOS_ERR_STATE, "the engine's device %d is not current (%d is)", e->device, dev);
+    // what nlos_step would allocate on the way
+    if (!e->probe_ucells.empty() && e->step + n > e->hist_cap)
+        return fail(NLOS_ERR_ARG, "probe history holds %lld steps, %lld requested without allocation "
+                    "(nlos_set_probes reserves %d; nlos_step grows it)", e->hist_cap, (long long)(e->step + n),
+                    NLOS_PROBE_HISTORY_RESERVE);
+    if ((flags & NLOS_STEP_CAPTURE_FIELD) && !e->field_alloc)
+        return fail(NLOS_ERR_STATE, "no field buffer yet: capture once with nlos_step before capturing here");
+    if (e->timing) return fail(NLOS_ERR_STATE, "timing records allocate events; disable it for nlos_step_noalloc");
+    const long long bound = e->has_source ? (e->src.injection == NLOS_INJECT_TRILINEAR ? 8 : MAX_STENCIL) : 1;
+    if (bound > NOALLOC_ENTRIES)
+        return fail(NLOS_ERR_ARG, "a step of %lld source entries exceeds the staging array of %lld",
+                    bound, NOALLOC_ENTRIES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    unsigned char *sp = nullptr;
+    if (int rc = scratch_address(dev, &sp)) return rc;
+    long long *d_cells = reinterpret_cast<long long *>(sp);
+    double *d_adds = reinterpret_cast<double *>(sp + NOALLOC_ENTRIES * 8);
+    const long long chunk = std::min(NOALLOC_CHUNK, NOALLOC_ENTRIES / bound);
+    for (long long base = 0; base < n; base += chunk) {
+        const long long m = std::min(chunk, n - base);
+        long long off = 0;
+        g_sched_off[0] = 0;
+        for (long long s = 0; s < m; ++s) {
+            // t = s*dt (05:1079); at most `bound` entries per step, so the chunk fits
+            off += schedule_cells(e, (double)(e->step + s) * e->dt, g_sched_cells + off, g_sched_adds + off);
+            g_sched_off[s + 1] = off;
+        }
+        if (off > 0) {
+            // the previous chunk's kernels on this stream read the staging array first
+            HIP_TRY(hipMemcpyAsync(d_cells, g_sched_cells, off * 8, hipMemcpyHostToDevice, e->stream));
+            HIP_TRY(hipMemcpyAsync(d_adds, g_sched_adds, off * 8, hipMemcpyHostToDevice, e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+        }
+        for (long long s = 0; s < m; ++s) {
+            if (int rc = launch_source(e, g_sched_off[s], g_sched_off[s + 1] - g_sched_off[s], d_cells, d_adds))
+                return rc;
+            if (int rc = launch_halos(e)) return rc;
+            const bool cap = (flags & NLOS_STEP_CAPTURE_FIELD) && base + s == n - 1;
+            if (int rc = launch_sweep(e, cap)) return rc;
+        }
     }
     return NLOS_OK;
 }

@@ -107,6 +107,7 @@ SIGNATURES = {
     "nlos_exchange_halos": (C.c_int, [_E]),
     "nlos_step_collision": (C.c_int, [_E]),
     "nlos_step": (C.c_int, [_E, C.c_int64, C.c_int32]),
+    "nlos_step_noalloc": (C.c_int, [_E, C.c_int64, C.c_int32]),
     "nlos_synchronize": (C.c_int, [_E]),
     "nlos_read_probes": (C.c_int, [_E, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "nlos_read_field": (C.c_int, [_E, _P]),
@@ -311,6 +312,10 @@ class Engine:
 
     def step(self, n: int, capture_field: bool = False) -> None:
         _check(self.lib.nlos_step(self.h, n, STEP_CAPTURE_FIELD if capture_field else 0))
+
+    def step_noalloc(self, n: int, capture_field: bool = False) -> None:
+        """nlos_step without device allocation (nlos_step_noalloc)."""
+        _check(self.lib.nlos_step_noalloc(self.h, n, STEP_CAPTURE_FIELD if capture_field else 0))
 
     def synchronize(self) -> None:
         _check(self.lib.nlos_synchronize(self.h))

@@ -235,6 +235,21 @@ int nlos_step_collision(nlos_engine *e);           /* 05:906-972 + probe record 
  * step for nlos_read_field. */
 #define NLOS_STEP_CAPTURE_FIELD 1
 int nlos_step(nlos_engine *e, int64_t n, int32_t flags);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * nlos_step_noalloc runs the same n steps as nlos_step, bit for bit, and calls no
+ * hipMalloc, hipFree or other allocating routine.  The per-step source schedule goes through
+ * one static device array per GPU of NLOS_NOALLOC_SOURCE_ENTRIES entries (4 MiB, held until
+ * the process ends) instead of the engine's growable buffers.  It refuses (engine
+ * unchanged) what would need an allocation: a probe history past its capacity
+ * (nlos_set_probes reserves NLOS_PROBE_HISTORY_RESERVE steps, nlos_step grows it),
+ * NLOS_STEP_CAPTURE_FIELD before the first capture by nlos_step, timing enabled, a family-A
+ * engine, or an engine whose device is not current.  Calls on one GPU share its array: two
+ * engines stepped this way on different streams at the same time race. */
+#define NLOS_NOALLOC_SOURCE_ENTRIES 262144
+#define NLOS_PROBE_HISTORY_RESERVE 4096
+int nlos_step_noalloc(nlos_engine *e, int64_t n, int32_t flags);
 int nlos_synchronize(nlos_engine *e);
 
 /* ---- results -------------------------------------------------------------------------- */
