@@ -480,39 +480,32 @@ def bench_frames(args) -> None:
 
 
 def bench_legacy04(args) -> None:
-    """The legacy family-A step (NLOS_FAMILY_A, 01/04 step_pull semantics, FP64) on
-    04_hpc_scalability.cpp's benchmark geometry: +-13 m box at dx = 0.125 (208^3 cells),
+    """The legacy family-A step (NLOS_FAMILY_A, 01/04 step_pull semantics, FP64) running
+    04_hpc_scalability.cpp's benchmark (04:360-404): +-13 m box at dx = 0.125 (208^3 cells),
     tau 0.51, a solid cylinder of radius 4 along z, 04's radial sponge (beyond r = 10 over 3,
-    strength 0.5), z periodic.  04's timed loop: 100 warm-up steps, 2000 timed, MLUPS over
-    all nodes (04:360-404).  04's moving source is left out (family-A sources are not
-    built); the run starts from a small density pulse instead.  CPU baseline: the oracle's
-    family-A restatement (OpenMP) on a bounded number of steps of the same grid."""
+    strength 0.5), z periodic, 04's line source (04:212-235) from rest; 100 warm-up steps,
+    then 2000 timed steps with the source time restarted at 0, MLUPS over all nodes.  CPU
+    baseline: the oracle's family-A restatement (OpenMP) on a bounded number of steps of the
+    same grid."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
-    import math
     import family_a as fa
     steps = args.steps if args.steps != 200 else 2000
     warmup = args.warmup if args.warmup != 20 else 100
     cfg = fa.config(x=(-13.0, 13.0), y=(-13.0, 13.0), z=(-13.0, 13.0), dx=0.125, tau=0.51)
     nx, ny, nz = fa.dims(cfg)
-    solid, rho = fa.cylinder_and_pulse(cfg, radius=4.0, centre=(0.0, -6.0), width=2.0, amp=0.01)
-    sg = np.zeros((ny, nx))
-    for j in range(ny):
-        for i in range(nx):
-            x = cfg.x_min + float(i) * cfg.dx + 0.5 * cfg.dx          # 04:187-189
-            y = cfg.y_min + float(j) * cfg.dx + 0.5 * cfg.dx
-            r = math.sqrt(x * x + y * y)
-            if r > 10.0:
-                ratio = min(1.0, (r - 10.0) / 3.0)
-                sg[j, i] = 0.5 * ratio * ratio
-    sigma = np.broadcast_to(sg, (nz, ny, nx)).copy()
+    solid = fa.cylinder_04(cfg)
+    sigma = fa.radial_sponge_04(cfg)
+    src = nlos.reference04_line_source(0)
     e = nlos.Engine(cfg)
     e.set_mask(solid)
-    e.set_density(rho)
     e.set_sponge(sigma)
+    e.set_line_source(src)
     e.set_probes([(0.0, 9.0, 0.0)])
     log(f"legacy04: {nx}x{ny}x{nz}, warmup {warmup}, timed {steps}")
     e.step(warmup)
     e.synchronize()
+    src.step0 = warmup                 # 04's timed loop counts t from 0 again (04:381-388)
+    e.set_line_source(src)
     e.timing(True)
     e.timing_reset()
     t = time.perf_counter()
@@ -520,6 +513,7 @@ def bench_legacy04(args) -> None:
     e.synchronize()
     wall = time.perf_counter() - t
     tm = e.timing(False)
+    probe = e.read_probes()
     cells = nx * ny * nz
     n_solid = int(solid.sum())
     mlups = cells * steps / wall / 1e6
@@ -531,8 +525,9 @@ def bench_legacy04(args) -> None:
            "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warmup,
            "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"04 geometry: {nx}x{ny}x{nz} cells, cylinder r=4, radial sponge, "
-                                  "z periodic, initial pulse instead of 04's source"},
+           "config": {"workload": f"04 benchmark: {nx}x{ny}x{nz} cells, cylinder r=4, radial sponge, "
+                                  "z periodic, 04's line source, 100 warm-up + timed steps"},
+           "probe_max_abs": float(np.abs(probe).max()),
            "sweep_ms": round(tm.avg_ms, 4),
            "roofline": {"bound": "hbm", "bytes_per_update": bpu,
                         "achieved": round(alg / (wall / steps) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -542,8 +537,8 @@ def bench_legacy04(args) -> None:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         o = OracleA(cfg)
         o.set_mask(solid)
-        o.set_density(rho)
         o.set_sponge(sigma)
+        o.set_line_source(nlos.reference04_line_source(0))
         o.run(1)
         n_cpu = 40                 # about 10-20 s on the host cores
         t = time.perf_counter()

@@ -272,6 +272,9 @@ struct nlos_engine {
     // legacy family-A mode (legacy_a.hip)
     int family = NLOS_FAMILY_B;
     DevBuf sigma_a;                   // sponge per interior cell (FP64)
+    nlos_line_source line{};          // 04's line source (amp 0: none)
+    DevBuf line_cells, line_spatial;  // its fluid cells and exp(-ln2 d2 / b) per cell
+    long long n_line = 0;
     DevBuf field_a;                   // rho per interior cell of the last capture step (FP64)
     std::vector<double> rho0;         // density set before the first step (t = 0 probe line)
 
@@ -633,6 +636,7 @@ int upload_schedule(nlos_engine *e) {
 
 // ---- launch helpers -------------------------------------------------------------------
 int sync_face_buffers(nlos_engine *e);
+int build_line_cells(nlos_engine *e);
 
 // Kernel arguments for the state `phase` steps ahead of the current one (0 = current).
 // A two-array step swaps the population arrays, and a step with tracked boundary solids
@@ -809,6 +813,13 @@ int launch_a(nlos_engine *e, bool capture) {
     if (capture) {
         if (e->dump_pending) HIP_TRY(hipStreamWaitEvent(e->stream, e->dump_copied, 0));
         HIP_TRY(e->field_a.reserve((size_t)e->n_interior() * 8));
+    }
+    if (e->n_line > 0) {
+        // t = s * dt (04:371, 383), the timed loop's s counted from step0
+        const double t = (double)(e->step - e->line.step0) * e->line.dt;
+        HIP_TRY(launch_a_line(static_cast<double *>(e->pop.p), e->g, e->line_cells.as<long long>(),
+                              e->line_spatial.as<double>(), e->n_line, e->line.amp, std::sin(e->line.omega * t),
+                              e->line.dt, e->stream));
     }
     HIP_TRY(launch_a_halo(static_cast<double *>(e->pop.p), e->g, e->stream));
     AStepArgs a{};
@@ -1063,7 +1074,7 @@ NLOS_API int nlos_destroy(nlos_engine *e) {
                       &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
                       &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
                       &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1], &e->wall_bits,
-                      &e->sigma_a, &e->field_a, &e->xh};
+                      &e->sigma_a, &e->field_a, &e->xh, &e->line_cells, &e->line_spatial};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : e->ck) b.release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
@@ -1229,6 +1240,8 @@ NLOS_API int nlos_set_mask(nlos_engine *e, const uint8_t *m) {
     HIP_TRY(hipMemcpy(e->mask.p, full.data(), full.size(), hipMemcpyHostToDevice));
     e->hmask.assign(m, m + e->n_interior());
     for (auto &b : e->hmask) b = b ? 1 : 0;
+    if (e->family == NLOS_FAMILY_A && e->line.amp != 0.0)
+        if (int rc = build_line_cells(e)) return rc;
     return finalize_mask(e);
 }
 
@@ -1246,6 +1259,51 @@ NLOS_API int nlos_set_sponge(nlos_engine *e, const double *sigma) {
     if (!sigma) return fail(NLOS_ERR_ARG, "null sponge");
     HIP_TRY(hipMemcpy(e->sigma_a.p, sigma, (size_t)e->n_interior() * 8, hipMemcpyHostToDevice));
     return NLOS_OK;
+}
+
+namespace {
+
+// The cells and spatial factors of 04's apply_source (04:220-232), in its operation order,
+// for the current mask (the reference tests the mask at the add).
+int build_line_cells(nlos_engine *e) {
+    const nlos_line_source &s = e->line;
+    std::vector<long long> cells;
+    std::vector<double> spatial;
+    if (s.amp != 0.0) {
+        const nlos_config &c = e->cfg;
+        const double ln2 = std::log(2.0);
+        for (int k = 0; k < e->g.nz; ++k)
+            for (int j = 0; j < e->g.ny; ++j)
+                for (int i = 0; i < e->g.nx; ++i) {
+                    const double x = c.x_min + (double)i * c.dx + 0.5 * c.dx;
+                    const double y = c.y_min + (double)j * c.dx + 0.5 * c.dx;
+                    const double d2 = (x - s.x0) * (x - s.x0) + (y - s.y0) * (y - s.y0);
+                    if (d2 > s.r2_max) continue;
+                    if (e->solid_interior(i, j, k)) continue;
+                    cells.push_back(e->cell(i + 1, j + 1, k + 1));
+                    spatial.push_back(std::exp(-ln2 * d2 / s.b));
+                }
+    }
+    if (!cells.empty()) {
+        HIP_TRY(e->line_cells.reserve(cells.size() * 8));
+        HIP_TRY(e->line_spatial.reserve(spatial.size() * 8));
+        HIP_TRY(hipMemcpy(e->line_cells.p, cells.data(), cells.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->line_spatial.p, spatial.data(), spatial.size() * 8, hipMemcpyHostToDevice));
+    }
+    e->n_line = (long long)cells.size();
+    return NLOS_OK;
+}
+
+}  // namespace
+
+NLOS_API int nlos_set_line_source(nlos_engine *e, const nlos_line_source *s) {
+    if (int rc = ready(e)) return rc;
+    if (!s) return fail(NLOS_ERR_ARG, "null line source");
+    if (e->family != NLOS_FAMILY_A) return fail(NLOS_ERR_UNSUPPORTED, "the line source is a NLOS_FAMILY_A feature");
+    if (s->amp != 0.0 && (!(s->b > 0.0) || !(s->dt > 0.0) || !(s->r2_max >= 0.0)))
+        return fail(NLOS_ERR_ARG, "line source needs b > 0, dt > 0 and r2_max >= 0");
+    e->line = *s;
+    return build_line_cells(e);
 }
 
 NLOS_API int nlos_set_density(nlos_engine *e, const double *rho) {

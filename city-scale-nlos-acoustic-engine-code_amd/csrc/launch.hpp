@@ -76,6 +76,9 @@ hipError_t launch_pull_pattern16(const void *in, void *out, const GridDesc &g, h
 // stream (01's step_pull).
 hipError_t launch_a_halo(double *f, const GridDesc &g, hipStream_t s);
 hipError_t launch_a_sweep(const AStepArgs &a, hipStream_t s);
+// 04's line source: f_q(cells[e]) += (w_q * ((amp * spatial[e]) * sin_t)) * dt, e < n.
+hipError_t launch_a_line(double *f, const GridDesc &g, const long long *cells, const double *spatial, long long n,
+                         double amp, double sin_t, double dt, hipStream_t s);
 
 // Voxeliser (voxel.hip).
 struct VoxelArgs {

@@ -1,6 +1,6 @@
 // Family-A step (01_acoustic_scattering.cpp / 04_hpc_scalability.cpp step_pull semantics)
 // for the engine's legacy mode, FP64 with contraction off so that every value rounds as in
-// the reference.  Four launches per step:
+// the reference.  Four launches per step (five with 04's line source, k_a_line, first):
 //   k_a_halo_xy   x and y halo cells at rest (no neighbour rank, 01:234-246)
 //   k_a_halo_z    z halo planes copied periodically, x/y halo columns included (01:283-300)
 //   k_a_collide   post-collision value of every fluid cell, halo included (01:315-343: the
@@ -147,6 +147,27 @@ __global__ __launch_bounds__(256) void k_a_stream(AStepArgs a) {
             if (v < c) lo = mid + 1; else hi = mid - 1;
         }
     }
+}
+
+// 04_hpc_scalability's apply_source (04:212-235) on the listed fluid cells: the value
+// (Amp * spatial) * sin(omega t) (04:229) and the add f += (w_q * value) * dt (04:233-236), in
+// the reference's operation order; sin(omega t) and the spatial factors come from the host
+// (glibc), so they round as the reference's.
+__global__ __launch_bounds__(256) void k_a_line(double *f, GridDesc g, const long long *cells, const double *spatial,
+                                                long long n, double amp, double sin_t, double dt) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long long c = cells[t];
+    const double v = amp * spatial[t] * sin_t;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[pop_index<M_F64>(q, c, g.ncell)] += W64[q] * v * dt;
+}
+
+hipError_t launch_a_line(double *f, const GridDesc &g, const long long *cells, const double *spatial, long long n,
+                         double amp, double sin_t, double dt, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_a_line<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(f, g, cells, spatial, n, amp, sin_t, dt);
+    return hipGetLastError();
 }
 
 hipError_t launch_a_halo(double *f, const GridDesc &g, hipStream_t s) {

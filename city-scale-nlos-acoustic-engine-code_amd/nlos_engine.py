@@ -78,6 +78,20 @@ class Grad(C.Structure):
                 ("d_samples", C.POINTER(C.c_double)), ("n_samples", C.c_int64)]
 
 
+class LineSource(C.Structure):
+    """nlos_line_source: 04_hpc_scalability's line source (family A)."""
+    _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("amp", C.c_double), ("b", C.c_double),
+                ("omega", C.c_double), ("r2_max", C.c_double), ("dt", C.c_double), ("step0", C.c_int64)]
+
+
+def reference04_line_source(step0: int = 0) -> LineSource:
+    """04:65-72 and 04:32-37: Amp 0.01, b 2, omega pi/15, centre (0, -6), cut-off 36 m^2,
+    dt = 0.577350269 dx / c with dx = 0.125, c = 1."""
+    import math
+    return LineSource(x0=0.0, y0=-6.0, amp=0.01, b=2.0, omega=math.pi / 15.0, r2_max=36.0,
+                      dt=0.577350269 * 0.125 / 1.0, step0=step0)
+
+
 class Tune(C.Structure):
     _fields_ = [("block_x", C.c_int32), ("block_y", C.c_int32), ("candidates", C.c_int32),
                 ("reserved", C.c_int32), ("best_ms", C.c_double)]
@@ -101,6 +115,7 @@ SIGNATURES = {
     "nlos_get_mask": (C.c_int, [_E, _P]),
     "nlos_set_density": (C.c_int, [_E, _P]),
     "nlos_set_sponge": (C.c_int, [_E, _P]),
+    "nlos_set_line_source": (C.c_int, [_E, C.POINTER(LineSource)]),
     "nlos_set_source": (C.c_int, [_E, C.POINTER(Source)]),
     "nlos_set_probes": (C.c_int, [_E, _P, C.c_int32, _P]),
     "nlos_apply_source": (C.c_int, [_E, C.c_double]),
@@ -285,6 +300,10 @@ class Engine:
         s = np.ascontiguousarray(sigma, dtype=np.float64)
         assert s.shape == tuple(reversed(self.dims))
         _check(self.lib.nlos_set_sponge(self.h, _ptr(s)))
+
+    def set_line_source(self, src: LineSource) -> None:
+        """FAMILY_A only: 04's line source (nlos_set_line_source)."""
+        _check(self.lib.nlos_set_line_source(self.h, C.byref(src)))
 
     def set_source(self, src: Source, samples: Optional[np.ndarray] = None) -> None:
         if samples is not None:
@@ -483,6 +502,20 @@ def reference05_source() -> Source:
     """Static Gaussian pulse of 05:1156-1164."""
     return default_source(sx=912.0, sy=66.6, sz=86.0, ex=912.0, ey=66.6, ez=86.0,
                           speed=0.0, amp=10.0, freq=0.0)
+
+
+def reference08_config(precision="f64") -> Config:
+    """The hard-coded configuration of 08_city_scale_uav.cpp: the member defaults of the box
+    (08:250-253), DX = 1.5 and the STL shift of its main (08:1066-1071)."""
+    return default_config(dx=1.5, x_min=0.0, x_max=1650.0, y_min=0.0, y_max=1150.0,
+                          z_min=0.0, z_max=250.0, sponge_width=20.0 * 1.5,
+                          world_offset=(22248.0, 4123.0, 14.0), precision=precision)
+
+
+def reference08_source() -> Source:
+    """The moving harmonic source of 08:1073-1082 (sine waveform, 08:543)."""
+    return default_source(sx=912.0, sy=66.6, sz=86.0, ex=912.0, ey=1127.3, ez=86.0,
+                          speed=30.0, amp=0.5, freq=20.0, waveform=WAVE_SINE)
 
 
 class Batch:

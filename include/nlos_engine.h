@@ -105,9 +105,9 @@ typedef struct nlos_config {
  *   The sponge is per cell: by default 01's rule on the x/y faces, with thickness
  *   sponge_width and strength sponge_strength (01:207-225); nlos_set_sponge replaces it
  *   (04's radial ramp).
- *   Needs NLOS_F64 and the two-array pattern.  Sources, NSCBC, the ground halo, batches
- *   and the adjoint are family-B features and are refused or ignored (ground and NSCBC
- *   are off).  nlos_write_vtk writes 01's format (01:414-436, scale 100);
+ *   Needs NLOS_F64 and the two-array pattern.  Its source is 04's line source
+ *   (nlos_set_line_source); nlos_set_source, NSCBC, the ground halo, batches and the
+ *   adjoint are family-B features and are refused or ignored (ground and NSCBC are off).  nlos_write_vtk writes 01's format (01:414-436, scale 100);
  *   nlos_write_probe_dat writes 01's files including the t = 0 line. */
 typedef enum nlos_family { NLOS_FAMILY_B = 0, NLOS_FAMILY_A = 1 } nlos_family;
 
@@ -217,6 +217,20 @@ int nlos_get_mask(nlos_engine *e, uint8_t *mask);
 /* NLOS_FAMILY_A only: sponge strength per interior cell, x fastest (replaces 01's default
  * rule; 04:176-208 is a radial one).  A cell blends when sigma > 1e-6 (01:313). */
 int nlos_set_sponge(nlos_engine *e, const double *sigma);
+/* NLOS_FAMILY_A only: the line source of 04_hpc_scalability.cpp (apply_source, 04:212-235),
+ * added to the populations at the start of every step, before the halo exchange
+ * (04:369-388).  Every fluid interior cell, in every z layer, whose centre
+ * (x, y) = (x_min + i dx + dx/2, y_min + j dx + dx/2) has d2 = (x - x0)^2 + (y - y0)^2 <= r2_max
+ * receives f_q += w_q v dt with v = amp exp(-ln 2 d2 / b) sin(omega t), t = (s - step0) dt at
+ * step s.  04's values: x0 = 0, y0 = -6, amp = 0.01, b = 2, omega = pi/15, r2_max = 36,
+ * dt = 0.577350269 dx / c (04:32-37, 65-72); its timed loop restarts t at 0 after 100 warm-up
+ * steps (step0 = 100 then).  Setting it again replaces it; amp = 0 switches it off. */
+typedef struct nlos_line_source {
+    double x0, y0;
+    double amp, b, omega, r2_max, dt;
+    int64_t step0;
+} nlos_line_source;
+int nlos_set_line_source(nlos_engine *e, const nlos_line_source *s);
 /* Initial populations f = feq(rho, 0) = w_q*rho on fluid cells (interior rho, x fastest). */
 int nlos_set_density(nlos_engine *e, const double *rho);
 int nlos_set_source(nlos_engine *e, const nlos_source *src);

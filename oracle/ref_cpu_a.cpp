@@ -17,6 +17,8 @@
 //     sponge of the destination, rho = sum f    01:303-359
 //   probe cell (int)((x-min)/dx - 0.5) + 1,
 //     rho-1 recorded before each step           01:448-479, 01:507-510
+//   04's line source, added before the halo
+//     exchange of every step                    04:212-235, 04:369-388
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -52,12 +54,35 @@ struct oracle_a {
     std::vector<size_t> probe_cells;   // index into the halo-padded grid, or SIZE_MAX
     std::vector<std::vector<double>> hist;
     long long steps = 0;
+    nlos_line_source line{};
 
     size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)j * nxt + (size_t)k * nxt * nyt; }
 
     void record() {
         for (size_t p = 0; p < probe_cells.size(); ++p)
             hist[p].push_back(probe_cells[p] == SIZE_MAX ? 0.0 : rho[probe_cells[p]] - 1.0);
+    }
+
+    // apply_source of 04 (04:212-235): t_phys = (s - step0) * dt as its loops count s
+    void apply_line_source() {
+        const nlos_line_source &L = line;
+        if (L.amp == 0.0) return;
+        const double t_phys = (double)(steps - L.step0) * L.dt;
+        const double dt_phys = L.dt;
+#pragma omp parallel for collapse(3)
+        for (int k = 1; k <= nz; ++k)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    double x = xmin + (double)(i - 1) * dx + 0.5 * dx;
+                    double y = ymin + (double)(j - 1) * dx + 0.5 * dx;
+                    double dist2 = (x - L.x0) * (x - L.x0) + (y - L.y0) * (y - L.y0);
+                    if (dist2 > L.r2_max) continue;
+                    double spatial = std::exp(-std::log(2.0) * dist2 / L.b);
+                    double source_val = L.amp * spatial * std::sin(L.omega * t_phys);
+                    size_t c = idx(i, j, k);
+                    if (!solid[c])
+                        for (int q = 0; q < Q; ++q) f[c * Q + q] += WA[q] * source_val * dt_phys;
+                }
     }
 
     void halos() {
@@ -229,12 +254,15 @@ int oracle_a_set_probes(oracle_a *o, const double *xyz, int32_t n) {
 void oracle_a_run(oracle_a *o, long long n) {
     for (long long s = 0; s < n; ++s) {
         if (o->steps == 0 && o->hist.size() && o->hist[0].empty()) o->record();
+        o->apply_line_source();
         o->halos();
         o->step();
         ++o->steps;
         o->record();
     }
 }
+
+void oracle_a_set_line_source(oracle_a *o, const nlos_line_source *s) { o->line = *s; }
 
 long long oracle_a_read_probes(const oracle_a *o, double *out) {
     long long m = 0;

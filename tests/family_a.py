@@ -60,3 +60,36 @@ def ref01_steps(cfg):
     """Records of 01's run loop (01:499-527): s = 0 .. (int)(11/dt) + 20."""
     dt = math.sqrt(1.0 / 3.0) * cfg.dx / cfg.c_phys
     return int(11.0 / dt) + 20
+
+
+def radial_sponge_04(cfg):
+    """04's init_physics sponge (04:176-208): beyond r = 10, ramp over 3, strength 0.5, with
+    04's cell centres x = X_MIN + i dx + 0.5 dx.  Array (nz, ny, nx)."""
+    nx, ny, nz = dims(cfg)
+    sg = np.zeros((ny, nx))
+    for j in range(ny):
+        for i in range(nx):
+            x = cfg.x_min + float(i) * cfg.dx + 0.5 * cfg.dx
+            y = cfg.y_min + float(j) * cfg.dx + 0.5 * cfg.dx
+            r = math.sqrt(x * x + y * y)
+            if r > 10.0:
+                ratio = min(1.0, (r - 10.0) / 3.0)
+                sg[j, i] = 0.5 * ratio * ratio
+    return np.broadcast_to(sg, (nz, ny, nx)).copy()
+
+
+def cylinder_04(cfg, radius=4.0):
+    """04's solid cylinder r < cyl_R along z (04:190-194), 04's cell centres."""
+    nx, ny, nz = dims(cfg)
+    plane = np.zeros((ny, nx), np.uint8)
+    for j in range(ny):
+        for i in range(nx):
+            x = cfg.x_min + float(i) * cfg.dx + 0.5 * cfg.dx
+            y = cfg.y_min + float(j) * cfg.dx + 0.5 * cfg.dx
+            plane[j, i] = 1 if math.sqrt(x * x + y * y) < radius else 0
+    return np.broadcast_to(plane, (nz, ny, nx)).copy()
+
+
+def coarse04_config(dx=0.5, z=(-1.5, 1.5)):
+    """04's +-13 m box and tau 0.51 (04:28-57) at a coarser dx."""
+    return config(x=(-13.0, 13.0), y=(-13.0, 13.0), z=z, dx=dx, tau=0.51)

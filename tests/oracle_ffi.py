@@ -196,6 +196,7 @@ def lib_a() -> C.CDLL:
             "oracle_a_dims": (None, [vp, vp]),
             "oracle_a_set_mask": (None, [vp, vp]),
             "oracle_a_set_sponge": (None, [vp, vp]),
+            "oracle_a_set_line_source": (None, [vp, C.POINTER(nlos.LineSource)]),
             "oracle_a_get_sponge": (None, [vp, vp]),
             "oracle_a_set_density": (None, [vp, vp]),
             "oracle_a_set_probes": (C.c_int, [vp, vp, C.c_int32]),
@@ -241,6 +242,9 @@ class OracleA:
     def set_sponge(self, sigma):
         s = np.ascontiguousarray(sigma, dtype=np.float64)
         self.L.oracle_a_set_sponge(self.h, _p(s))
+
+    def set_line_source(self, src):
+        self.L.oracle_a_set_line_source(self.h, C.byref(src))
 
     def sponge(self):
         nx, ny, nz = self.dims

@@ -169,3 +169,52 @@ def test_checkpoint_of_the_other_family_is_refused(gpu):
     with pytest.raises(nlos.EngineError) as ex:
         b.restore(blob)
     assert ex.value.code == 1                    # NLOS_ERR_ARG
+
+
+def test_line_source_of_04_bitwise_against_oracle(gpu):
+    """04's scene at dx = 0.5 (cylinder, radial sponge, z periodic) driven by 04's line
+    source, with 04's restart of the source time after a warm-up (04:369-388: t counts
+    from 0 again for the timed loop): probes and populations equal the oracle's bit for bit,
+    and every z layer stays identical."""
+    cfg = fa.coarse04_config()
+    solid, sigma = fa.cylinder_04(cfg), fa.radial_sponge_04(cfg)
+    src = nlos.reference04_line_source(0)
+    src.dt = 0.577350269 * cfg.dx / 1.0
+    pts = [(0.0, 9.0, 0.0), (7.1, -2.3, 0.4), (-3.3, -11.1, -1.0)]
+    print("[family A] 04 line source at dx=0.5: 30 + 30 steps", flush=True)
+    e = _engine(cfg, solid, None, None, pts)
+    e.set_sponge(sigma)
+    e.set_line_source(src)
+    o = OracleA(cfg)
+    o.set_mask(solid)
+    o.set_sponge(sigma)
+    o.set_probes(pts)
+    o.set_line_source(src)
+    e.step(30)
+    o.run(30)
+    src.step0 = 30
+    e.set_line_source(src)
+    o.set_line_source(src)
+    e.step(30, capture_field=True)
+    o.run(30)
+    e.synchronize()
+    got, want = e.read_probes(), o.read_probes()[:, 1:]      # the oracle also keeps s = 0
+    assert got.shape == (3, 60)
+    assert np.abs(want).max() > 1e-6
+    assert np.array_equal(got, want), float(np.abs(got - want).max())
+    f = e.debug_read_f()
+    fo = o.read_f()
+    fluid = solid == 0
+    assert np.array_equal(f[fluid], fo[fluid])
+    field = e.read_field()
+    for k in range(1, field.shape[0]):
+        assert np.array_equal(field[k], field[0]), k
+    e.close()
+
+
+def test_line_source_is_refused_in_family_b(gpu):
+    cfg = nlos.default_config(x_max=16.0, y_max=16.0, z_max=8.0, precision="f64")
+    e = nlos.Engine(cfg)
+    with pytest.raises(nlos.EngineError):
+        e.set_line_source(nlos.reference04_line_source())
+    e.close()
