@@ -2016,6 +2016,7 @@ struct nlos_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<long long> baked;         // launch shape the graph was captured with
     long long step = 0;
+    bool use_graph = true;                // NLOS_BATCH_GRAPH=0: launch every step directly
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double timed_ms = 0.0;
@@ -2222,6 +2223,7 @@ NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **
     auto bail = [&](int rc) { nlos_batch_destroy(b); return rc; };
     b->mode = cfgs[0].precision;
     if (const char *x = std::getenv("NLOS_BATCH_BLOCK")) parse_block(x, b->block);
+    if (const char *x = std::getenv("NLOS_BATCH_GRAPH")) b->use_graph = std::atoi(x) != 0;
     if (hipGetDevice(&b->device) != hipSuccess ||
         hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
         return bail(fail(NLOS_ERR_HIP, "device query failed"));
@@ -2360,12 +2362,20 @@ NLOS_API int nlos_batch_step(nlos_batch *b, int64_t n) {
             (long long)b->block.x, (long long)b->block.y, b->total_tiles,
             b->total_halo, b->total_solid, b->max_entries};
         if (moved || shape != b->baked) drop_graph(b);
-        if (m >= 2 && !b->exec) {
+        if (m >= 2 && !b->exec && b->use_graph) {
             if (int rc = batch_capture(b)) return rc;
             b->baked = shape;
         }
         if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->stream));
-        for (long long r = 0; r < m / 2; ++r) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+        for (long long r = 0; r < m / 2; ++r) {
+            if (b->use_graph) {
+                HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+            } else {
+                // the two captured steps, launched one by one
+                if (int rc = batch_launch_step(b, 0)) return rc;
+                if (int rc = batch_launch_step(b, 1)) return rc;
+            }
+        }
         if (m & 1) {
             if (int rc = batch_launch_step(b, 0)) return rc;
         }

@@ -128,3 +128,22 @@ def test_scene_reads_wait_for_the_batch(gpu):
         assert got[i][0].shape == want_p.shape == (len(d[3]), 300)
         assert np.array_equal(got[i][0], want_p), i
         assert np.array_equal(got[i][1], want_f, equal_nan=True), i
+
+
+def test_direct_launches_equal_graph_replays(gpu, monkeypatch):
+    """NLOS_BATCH_GRAPH=0 launches every step's kernels directly instead of replaying the
+    captured two-step graph (for tools that cannot trace graph launches): the same bits."""
+    defs = _defs("f32")
+    runs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("NLOS_BATCH_GRAPH", flag)
+        b = nlos.Batch([d[0] for d in defs])
+        for i, d in enumerate(defs):
+            _setup(b.scene(i), d)
+        b.step(9)
+        b.synchronize()
+        runs.append([(b.scene(i).read_probes(), b.scene(i).debug_read_f()) for i in range(len(defs))])
+        b.close()
+    for (pa, fa), (pb, fb) in zip(*runs):
+        assert np.array_equal(pa, pb)
+        assert np.array_equal(fa, fb, equal_nan=True)
