@@ -202,10 +202,13 @@ struct SourceArgs {
     float scale, inv_scale;
 };
 
-// Discrete adjoint of one two-array step (open faces at the rest state, no NSCBC).
-// state holds S'_n, the populations the step pulled from (after the source add); lam_in
-// the adjoint of S_{n+1} (what the step wrote), lam_out receives the adjoint of S'_n.
-// Both adjoint arrays are [q][ncell] in the compute type.
+// Discrete adjoint of one two-array step.  state holds S'_n, the populations the step
+// pulled from (after the source add, with the ghost values of that step in the halo);
+// lam_in the adjoint of S_{n+1} (what the step wrote), lam_out receives the adjoint of
+// S'_n.  Both adjoint arrays are [q][ncell] in the compute type.  On NSCBC faces
+// (nscbc_faces) the adjoint of a pulled ghost slot is left in that halo slot of lam_out;
+// the ghost pass (k_adjoint_ghost, face by face) then moves it to the boundary cell the
+// ghost was computed from and clears it.
 struct AdjArgs {
     const void *state;
     const void *lam_in;
@@ -220,6 +223,8 @@ struct AdjArgs {
     const long long *probe_cells;         // sorted unique probe cells
     int n_probe_cells;
     const double *dl_dp;                  // dL/d(rho-1) of this step per unique probe cell, or null
+    int nscbc_faces;                      // faces whose halo holds NSCBC ghosts (F_XM ...)
+    int face;                             // k_adjoint_ghost: the face it transposes
 };
 
 // One step of the legacy family-A mode (legacy_a.hip).  f holds the populations in

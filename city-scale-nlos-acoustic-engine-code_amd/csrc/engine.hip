@@ -2435,7 +2435,9 @@ namespace {
 int adjoint_supported(const nlos_engine *e) {
     if (e->pattern != NLOS_PATTERN_TWO_ARRAY)
         return fail(NLOS_ERR_UNSUPPORTED, "the adjoint needs the two-array stream pattern");
-    if (e->cfg.nscbc_faces != 0) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint needs nscbc_faces == 0");
+    if (e->cfg.nscbc_faces != 0 && e->n_solid > 0)
+        return fail(NLOS_ERR_UNSUPPORTED, "the adjoint through NSCBC faces needs no solid cells on their boundary "
+                    "planes (%d here)", e->n_solid);
     if (e->mode == M_F16 || e->mode == M_B16)
         return fail(NLOS_ERR_UNSUPPORTED, "the adjoint supports NLOS_F64 and NLOS_F32");
     if (e->family == NLOS_FAMILY_A) return fail(NLOS_ERR_UNSUPPORTED, "the adjoint is a NLOS_FAMILY_B feature");
@@ -2589,10 +2591,18 @@ NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_
             a.probe_cells = e->probe_cells.as<long long>();
             a.n_probe_cells = (int)nu;
             a.dl_dp = nu ? dl_dev.as<double>() + (size_t)(n - e->rec_start) * nu : nullptr;
+            a.nscbc_faces = e->cfg.nscbc_faces;
             const int cnt = (int)(off[k + 1] - off[k]);
             rc = hip(by_mode(e->mode, [&](auto L) {
                 using LL = decltype(L);
                 hipError_t r = LL::adjoint(a, e->stream);
+                // the ghosts of this step, transposed onto their boundary cells (05:684-835)
+                for (int f = 0; f < 6 && r == hipSuccess; ++f) {
+                    if (!((e->cfg.nscbc_faces >> f) & 1)) continue;
+                    AdjArgs ag = a;
+                    ag.face = f;
+                    r = LL::adjoint_ghost(ag, e->stream);
+                }
                 if (r == hipSuccess && cnt > 0)
                     r = LL::source_dot(lam[cur ^ 1].p, e->g.ncell, src_cells.as<long long>() + off[k], cnt,
                                        lam_src.as<double>() + off[k], e->stream);

@@ -927,14 +927,130 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
             lo[pop_index<MODE>(INV[q], c, N)] = (C)a.alpha * mu;
         } else {
             const int si = i - EX[q], sj = j - EY[q], sk = k - EZ[q];
-            if (si >= 1 && si <= g.nx && sj >= 1 && sj <= g.ny && sk >= 1 && sk <= g.nz)
+            const int out = (si < 1 || si > g.nx) + (sj < 1 || sj > g.ny) + (sk < 1 || sk > g.nz);
+            if (out == 0) {
                 lo[pop_index<MODE>(q, c - nb_off(g, q), N)] = mu;
+            } else if (out == 1 && a.nscbc_faces) {
+                // a ghost on a face cell of an NSCBC face: its adjoint waits in the halo slot
+                const int face = si < 1 ? 0 : si > g.nx ? 1 : sj < 1 ? 2 : sj > g.ny ? 3 : sk < 1 ? 4 : 5;
+                if (((a.nscbc_faces >> face) & 1) && !(a.mask[c - nb_off(g, q)] & MK_SOLID))
+                    lo[pop_index<MODE>(q, c - nb_off(g, q), N)] = mu;
+            }
         }
         // slot q of c is read only by the halo cell c + e_q (a constant): zero it
         const int ti = i + EX[q], tj = j + EY[q], tk = k + EZ[q];
         if (q != 0 && (ti < 1 || ti > g.nx || tj < 1 || tj > g.ny || tk < 1 || tk > g.nz) &&
             !(a.mask[c + nb_off(g, q)] & MK_SOLID))
             lo[pop_index<MODE>(q, c, N)] = C(0);
+    }
+}
+
+// Transposed NSCBC ghost fill of one face (05:684-835).  Forward, per face cell h of the
+// halo and its interior neighbour I: (rho_I, u_I) = moments of f_I; on a minus face
+// W = u_I,a - cs (rho_I - 1), rho_B = 1 - W / (2 cs), u_B,a = W / 2 (plus face: W = u_I,a +
+// cs (rho_I - 1), rho_B = 1 + W / (2 cs)), tangential u_B = u_I; ghost_q = feq_q(rho_B, u_B)
+// + f_I,q - feq_q(rho_I, u_I) for the slots interior cells pull.  The transpose sends the
+// adjoint g_q of those slots to I:
+//   lam_I,p += g_p + d_rho + e_p . d_j,
+// with (d_rho, d_j) the adjoint of the moments, through
+//   dF/drho = w (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2), dF/du = w rho (3 e + 9 (e.u) e - 3 u),
+//   d rho_B/d rho_I = 1/2, d rho_B/d u_I,a = -+ 1/(2 cs), d u_B,a/d rho_I = -+ cs/2,
+//   d u_B,a/d u_I,a = 1/2, u_I = j / rho_I.
+// One thread per face cell; I cells of one face are distinct, so faces run one per launch.
+template <class C>
+__device__ __forceinline__ void feq_partials(int q, C r, C ux, C uy, C uz, C &dr, C &du0, C &du1, C &du2) {
+    const C eu = (C)EX[q] * ux + (C)EY[q] * uy + (C)EZ[q] * uz;
+    const C w = wq<C>(q);
+    dr = w * (C(1) + C(3) * eu + C(4.5) * eu * eu - C(1.5) * (ux * ux + uy * uy + uz * uz));
+    const C s = w * r;
+    du0 = s * (C(3) * (C)EX[q] + C(9) * eu * (C)EX[q] - C(3) * ux);
+    du1 = s * (C(3) * (C)EY[q] + C(9) * eu * (C)EY[q] - C(3) * uy);
+    du2 = s * (C(3) * (C)EZ[q] + C(9) * eu * (C)EZ[q] - C(3) * uz);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_adjoint_ghost(AdjArgs a) {
+    using C = typename Tr<MODE>::C;
+    const GridDesc &g = a.g;
+    const int axis = a.face >> 1;
+    const bool plus = a.face & 1;
+    const int n1 = axis == 0 ? g.ny : g.nx;
+    const int n2 = axis == 2 ? g.ny : g.nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n1 * n2) return;
+    const int u = (int)(t % n1) + 1, v = (int)(t / n1) + 1;
+    int hi, hj, hk;
+    if (axis == 0) { hi = plus ? g.nx + 1 : 0; hj = u; hk = v; }
+    else if (axis == 1) { hi = u; hj = plus ? g.ny + 1 : 0; hk = v; }
+    else { hi = u; hj = v; hk = plus ? g.nz + 1 : 0; }
+    const int d = plus ? -1 : 1;   // step from the halo cell to I
+    const long long h = cell_index(g, hi, hj, hk);
+    const long long I = cell_index(g, hi + (axis == 0 ? d : 0), hj + (axis == 1 ? d : 0), hk + (axis == 2 ? d : 0));
+    if ((a.mask[h] & MK_SOLID) || (a.mask[I] & MK_SOLID)) return;
+    const long long N = g.ncell;
+    C *lam = static_cast<C *>(a.lam_out);
+    // adjoint of the pulled ghost slots (target inside the interior), cleared once read
+    C gl[Q];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        gl[q] = C(0);
+        const int ti = hi + EX[q], tj = hj + EY[q], tk = hk + EZ[q];
+        if (q == 0 || ti < 1 || ti > g.nx || tj < 1 || tj > g.ny || tk < 1 || tk > g.nz) continue;
+        const long long idx = pop_index<MODE>(q, h, N);
+        gl[q] = lam[idx];
+        lam[idx] = C(0);
+        any = any || gl[q] != C(0);
+    }
+    if (!any) return;
+    // forward boundary state from S'_n(I)
+    C f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        C v0 = ld<MODE>(a.state, pop_index<MODE>(q, I, N), a.inv_scale);
+        if constexpr (MODE != M_F64) v0 += wq<C>(q);
+        f[q] = v0;
+    }
+    C r = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        r += f[q];
+        jx += f[q] * (C)EX[q];
+        jy += f[q] * (C)EY[q];
+        jz += f[q] * (C)EZ[q];
+    }
+    const C uI[3] = {jx / r, jy / r, jz / r};
+    const C cs = (C)CS;
+    const C sg = plus ? C(1) : C(-1);
+    const C W = uI[axis] + sg * cs * (r - C(1));
+    const C rB = C(1) + sg * W / (C(2) * cs);
+    C uB[3] = {uI[0], uI[1], uI[2]};
+    uB[axis] = W / C(2);
+    // adjoints of (rho_B, u_B) and (rho_I, u_I) from the feq terms
+    C aB = 0, vB[3] = {0, 0, 0}, aI = 0, vI[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        if (gl[q] == C(0)) continue;
+        C dr, d0, d1, d2;
+        feq_partials<C>(q, rB, uB[0], uB[1], uB[2], dr, d0, d1, d2);
+        aB += gl[q] * dr; vB[0] += gl[q] * d0; vB[1] += gl[q] * d1; vB[2] += gl[q] * d2;
+        feq_partials<C>(q, r, uI[0], uI[1], uI[2], dr, d0, d1, d2);
+        aI += gl[q] * dr; vI[0] += gl[q] * d0; vI[1] += gl[q] * d1; vI[2] += gl[q] * d2;
+    }
+    // through rho_B(rho_I, u_I,a), u_B,a(rho_I, u_I,a) and u_B,t = u_I,t
+    C dRho = aB * C(0.5) + vB[axis] * (sg * cs * C(0.5)) - aI;
+    C dU[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dU[c] = vB[c] - vI[c];
+    dU[axis] = aB * (sg / (C(2) * cs)) + vB[axis] * C(0.5) - vI[axis];
+    // through u_I = j / rho_I
+    const C ir = C(1) / r;
+    const C dM0 = dRho - (dU[0] * uI[0] + dU[1] * uI[1] + dU[2] * uI[2]) * ir;
+    const C dJ[3] = {dU[0] * ir, dU[1] * ir, dU[2] * ir};
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+        const long long idx = pop_index<MODE>(p, I, N);
+        lam[idx] += gl[p] + dM0 + (C)EX[p] * dJ[0] + (C)EY[p] * dJ[1] + (C)EZ[p] * dJ[2];
     }
 }
 
@@ -1610,6 +1726,18 @@ hipError_t Launch<MODE>::adjoint(const AdjArgs &a, hipStream_t s) {
     } else {
         const long long total = (long long)a.g.nx * a.g.ny * a.g.nz;
         k_adjoint<MODE><<<blocks_for(total, 256), 256, 0, s>>>(a);
+        return hipGetLastError();
+    }
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::adjoint_ghost(const AdjArgs &a, hipStream_t s) {
+    if constexpr (MODE == M_B16 || MODE == M_F16) {
+        return hipErrorNotSupported;
+    } else {
+        const int axis = a.face >> 1;
+        const long long n = (long long)(axis == 0 ? a.g.ny : a.g.nx) * (axis == 2 ? a.g.ny : a.g.nz);
+        k_adjoint_ghost<MODE><<<blocks_for(n, 256), 256, 0, s>>>(a);
         return hipGetLastError();
     }
 }

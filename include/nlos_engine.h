@@ -317,9 +317,11 @@ int nlos_timing_get(nlos_engine *e, nlos_timing *t);
 /* ---- gradients (discrete adjoint) ------------------------------------------------------
  * No counterpart in the reference, which has no differentiation.  The adjoint transposes
  * the two-array step exactly (pull with half-way bounce-back and wall reflection, BGK
- * collision, sponge blend, probe record, source injection) for open faces held at the rest
- * state: it needs stream_pattern NLOS_PATTERN_TWO_ARRAY, nscbc_faces == 0 and precision
- * NLOS_F64 or NLOS_F32.
+ * collision, sponge blend, probe record, source injection) and the NSCBC ghost fill of the
+ * open faces (05:684-835); faces without NSCBC hold the rest state.  It needs
+ * stream_pattern NLOS_PATTERN_TWO_ARRAY, precision NLOS_F64 or NLOS_F32, and, with NSCBC
+ * faces, no solid cell on an NSCBC boundary plane (their tracked state is not transposed;
+ * NLOS_ERR_UNSUPPORTED).
  *
  * nlos_record_forward runs n steps like nlos_step and keeps the state at its first step and
  * every `every` steps after (device memory, one population array each).

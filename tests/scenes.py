@@ -72,14 +72,19 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
 
 
-def canyon_scene(precision="f64", nx=28, ny=24, nz=14, waveform="ricker", n_samples=0):
+def canyon_scene(precision="f64", nx=28, ny=24, nz=14, waveform="ricker", n_samples=0, open_faces=False):
     """Street canyon for the adjoint: two building rows along x with a 6-cell street, ground,
     sponge on five faces, open faces at the rest state (no NSCBC), partially reflecting walls,
-    a static Gaussian-stencil source (sigma 2 cells, radius 3) in the street, four probes."""
+    a static Gaussian-stencil source (sigma 2 cells, radius 3) in the street, four probes.
+    open_faces: NSCBC on the reference's five faces and no sponge, so that the waves leave
+    through the characteristic ghosts, and two more probes next to the x- and y+ faces."""
     cfg = nlos.default_config(dx=1.0, x_min=0.0, x_max=float(nx), y_min=0.0, y_max=float(ny),
                               z_min=0.0, z_max=float(nz), sponge_width=4.0, sponge_strength=0.2,
                               tau=0.6, sky_z=1e9, wall_reflect=0.9, nscbc_faces=0,
                               stream_pattern="two_array", precision=precision)
+    if open_faces:
+        cfg.nscbc_faces = nlos.FACES_REFERENCE
+        cfg.sponge_faces = 0
     solid = np.zeros((nz, ny, nx), np.uint8)
     c = ny // 2
     solid[:7, c - 3 - 5:c - 3, 4:nx - 4] = 1
@@ -97,4 +102,6 @@ def canyon_scene(precision="f64", nx=28, ny=24, nz=14, waveform="ricker", n_samp
         src = nlos.default_source(waveform=nlos.WAVE_RICKER, freq=90.0, t0=0.012, **kw)
         src_samples = None
     probes = [(3.5, c + 0.5, 1.5), (nx - 3.5, c - 1.5, 2.5), (15.5, c + 1.5, 10.5), (20.2, c - 0.4, 4.4)]
+    if open_faces:
+        probes += [(0.5, c + 0.5, 3.5), (13.5, ny - 0.5, 11.5)]
     return cfg, solid, src, src_samples, probes
