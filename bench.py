@@ -431,6 +431,49 @@ def bench_adjoint(args) -> None:
            "fd32_central": {"d_position_x": fd32_x, "d_amp": fd32_amp, "h_position_cells": h, "h_amp": ha,
                             "rel_err_x": abs(gp[0] - fd32_x) / abs(fd32_x) if fd32_x else None,
                             "rel_err_amp": abs(g["d_amp"] - fd32_amp) / abs(fd32_amp) if fd32_amp else None}}
+    # the same problem behind the reference's open boundary: NSCBC ghosts on five faces,
+    # transposed by the adjoint (k_adjoint_ghost)
+    wn = workloads.CanyonWorkload(n_samples=steps, nscbc=True)
+
+    def engine_n(precision, s, samp=samples):
+        e = nlos.Engine(wn.config(precision))
+        e.voxelize_stl(stl)
+        e.set_source(s, samp)
+        e.set_probes(probes)
+        return e
+
+    def loss_n(precision, s):
+        e = engine_n(precision, s)
+        e.step(steps)
+        r = e.read_probes() - obs
+        e.close()
+        return float(np.sum(r * r))
+
+    e = engine_n("f32", src)
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.synchronize()
+    tn_fwd = time.perf_counter() - t0
+    e.close()
+    e = engine_n("f32", src)
+    t0 = time.perf_counter()
+    e.record_forward(steps, every)
+    e.synchronize()
+    tn_rec = time.perf_counter() - t0
+    r = e.read_probes() - obs
+    t0 = time.perf_counter()
+    gn = e.backward(2.0 * r, n_samples=len(samples))
+    tn_bwd = time.perf_counter() - t0
+    e.close()
+    fdn_amp = fd4(lambda x: loss_n("f64", moved(amp=src.amp + x)), 1e-2 * src.amp)
+    fdn_x = fd4(lambda x: loss_n("f64", moved((x, 0.0, 0.0))), 1e-3)
+    log(f"nscbc: forward {tn_fwd:.2f} s, recorded forward {tn_rec:.2f} s, backward {tn_bwd:.2f} s")
+    out["nscbc_faces"] = {"faces": "x-, x+, y-, y+, z+ (the reference's)", "forward_s": round(tn_fwd, 3),
+                          "recorded_forward_s": round(tn_rec, 3), "backward_s": round(tn_bwd, 3),
+                          "backward_over_forward": round(tn_bwd / tn_fwd, 3),
+                          "gradient_total_over_forward": round((tn_rec + tn_bwd) / tn_fwd, 3),
+                          "rel_err_vs_fd64": {"d_amp": abs(gn["d_amp"] - fdn_amp) / abs(fdn_amp),
+                                              "d_position_x": abs(gn["d_position"][0] - fdn_x) / abs(fdn_x)}}
     print(json.dumps(out), flush=True)
 
 

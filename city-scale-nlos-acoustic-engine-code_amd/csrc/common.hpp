@@ -207,8 +207,8 @@ struct SourceArgs {
 // lam_in the adjoint of S_{n+1} (what the step wrote), lam_out receives the adjoint of
 // S'_n.  Both adjoint arrays are [q][ncell] in the compute type.  On NSCBC faces
 // (nscbc_faces) the adjoint of a pulled ghost slot is left in that halo slot of lam_out;
-// the ghost pass (k_adjoint_ghost, face by face) then moves it to the boundary cell the
-// ghost was computed from and clears it.
+// the ghost pass (k_adjoint_ghost, one thread per boundary cell) then moves it to the
+// boundary cell the ghost was computed from and clears it.
 struct AdjArgs {
     const void *state;
     const void *lam_in;
@@ -224,7 +224,6 @@ struct AdjArgs {
     int n_probe_cells;
     const double *dl_dp;                  // dL/d(rho-1) of this step per unique probe cell, or null
     int nscbc_faces;                      // faces whose halo holds NSCBC ghosts (F_XM ...)
-    int face;                             // k_adjoint_ghost: the face it transposes
 };
 
 // One step of the legacy family-A mode (legacy_a.hip).  f holds the populations in

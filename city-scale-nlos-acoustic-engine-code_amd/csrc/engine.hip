@@ -2597,12 +2597,7 @@ NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_
                 using LL = decltype(L);
                 hipError_t r = LL::adjoint(a, e->stream);
                 // the ghosts of this step, transposed onto their boundary cells (05:684-835)
-                for (int f = 0; f < 6 && r == hipSuccess; ++f) {
-                    if (!((e->cfg.nscbc_faces >> f) & 1)) continue;
-                    AdjArgs ag = a;
-                    ag.face = f;
-                    r = LL::adjoint_ghost(ag, e->stream);
-                }
+                if (r == hipSuccess && e->cfg.nscbc_faces) r = LL::adjoint_ghost(a, e->stream);
                 if (r == hipSuccess && cnt > 0)
                     r = LL::source_dot(lam[cur ^ 1].p, e->g.ncell, src_cells.as<long long>() + off[k], cnt,
                                        lam_src.as<double>() + off[k], e->stream);

@@ -854,13 +854,15 @@ __global__ __launch_bounds__(256) void k_prime_walls(void *pop, const uint8_t *m
 template <int MODE>
 __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
     using C = typename Tr<MODE>::C;
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const GridDesc &g = a.g;
-    const long long total = (long long)g.nx * g.ny * g.nz;
-    if (n >= total) return;
-    const int i = (int)(n % g.nx) + 1;
-    const int j = (int)((n / g.nx) % g.ny) + 1;
-    const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
+    // 256-cell row segments, 32-bit tile arithmetic (as the forward sweep)
+    const int tiles_x = (g.nx + 255) >> 8;
+    const int b = blockIdx.x;
+    const int row = b / tiles_x;
+    const int i = (b - row * tiles_x) * 256 + (int)threadIdx.x + 1;
+    if (i > g.nx) return;
+    const int j = row % g.ny + 1;
+    const int k = row / g.ny + 1;
     const long long c = cell_index(g, i, j, k);
     const long long N = g.ncell;
     C *lo = static_cast<C *>(a.lam_out);
@@ -956,7 +958,8 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
 //   dF/drho = w (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2), dF/du = w rho (3 e + 9 (e.u) e - 3 u),
 //   d rho_B/d rho_I = 1/2, d rho_B/d u_I,a = -+ 1/(2 cs), d u_B,a/d rho_I = -+ cs/2,
 //   d u_B,a/d u_I,a = 1/2, u_I = j / rho_I.
-// One thread per face cell; I cells of one face are distinct, so faces run one per launch.
+// One thread per boundary cell I, which adds the transposes of every NSCBC ghost it feeds
+// (up to three at an edge or corner), so no two threads write one cell.
 template <class C>
 __device__ __forceinline__ void feq_partials(int q, C r, C ux, C uy, C uz, C &dr, C &du0, C &du1, C &du2) {
     const C eu = (C)EX[q] * ux + (C)EY[q] * uy + (C)EZ[q] * uz;
@@ -972,37 +975,47 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_adjoint_ghost(AdjArgs a) {
     using C = typename Tr<MODE>::C;
     const GridDesc &g = a.g;
-    const int axis = a.face >> 1;
-    const bool plus = a.face & 1;
-    const int n1 = axis == 0 ? g.ny : g.nx;
-    const int n2 = axis == 2 ? g.ny : g.nz;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n1 * n2) return;
-    const int u = (int)(t % n1) + 1, v = (int)(t / n1) + 1;
-    int hi, hj, hk;
-    if (axis == 0) { hi = plus ? g.nx + 1 : 0; hj = u; hk = v; }
-    else if (axis == 1) { hi = u; hj = plus ? g.ny + 1 : 0; hk = v; }
-    else { hi = u; hj = v; hk = plus ? g.nz + 1 : 0; }
-    const int d = plus ? -1 : 1;   // step from the halo cell to I
-    const long long h = cell_index(g, hi, hj, hk);
-    const long long I = cell_index(g, hi + (axis == 0 ? d : 0), hj + (axis == 1 ? d : 0), hk + (axis == 2 ? d : 0));
-    if ((a.mask[h] & MK_SOLID) || (a.mask[I] & MK_SOLID)) return;
-    const long long N = g.ncell;
-    C *lam = static_cast<C *>(a.lam_out);
-    // adjoint of the pulled ghost slots (target inside the interior), cleared once read
-    C gl[Q];
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        gl[q] = C(0);
-        const int ti = hi + EX[q], tj = hj + EY[q], tk = hk + EZ[q];
-        if (q == 0 || ti < 1 || ti > g.nx || tj < 1 || tj > g.ny || tk < 1 || tk > g.nz) continue;
-        const long long idx = pop_index<MODE>(q, h, N);
-        gl[q] = lam[idx];
-        lam[idx] = C(0);
-        any = any || gl[q] != C(0);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    // one thread per cell of the interior's outer shell, each counted once: the x-face
+    // layers, then the y-face rows between them, then the z-face cells inside both
+    const long long nxs = nx > 1 ? 2 : 1, nys = ny > 1 ? 2 : 1, nzs = nz > 1 ? 2 : 1;
+    const long long ix = (long long)nx > 2 ? nx - 2 : 0, iy = (long long)ny > 2 ? ny - 2 : 0;
+    const long long cx = nxs * ny * nz, cy = nys * ix * nz, cz = nzs * ix * iy;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (t < cx) {
+        i = (t % nxs) ? nx : 1;
+        t /= nxs;
+        j = (int)(t % ny) + 1;
+        k = (int)(t / ny) + 1;
+    } else if ((t -= cx) < cy) {
+        j = (t % nys) ? ny : 1;
+        t /= nys;
+        i = (int)(t % ix) + 2;
+        k = (int)(t / ix) + 1;
+    } else if ((t -= cy) < cz) {
+        k = (t % nzs) ? nz : 1;
+        t /= nzs;
+        i = (int)(t % ix) + 2;
+        j = (int)(t / ix) + 2;
+    } else {
+        return;
     }
-    if (!any) return;
+    const long long N = g.ncell;
+    const long long I = cell_index(g, i, j, k);
+    if (a.mask[I] & MK_SOLID) return;
+    C *lam = static_cast<C *>(a.lam_out);
+    // the NSCBC faces this cell borders whose halo cell is fluid
+    const int at[6] = {i == 1, i == nx, j == 1, j == ny, k == 1, k == nz};
+    int faces = 0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        if (!at[f] || !((a.nscbc_faces >> f) & 1)) continue;
+        const int ax = f >> 1, d = (f & 1) ? 1 : -1;
+        const long long h = I + (ax == 0 ? d : ax == 1 ? (long long)d * g.px : (long long)d * g.plane);
+        if (!(a.mask[h] & MK_SOLID)) faces |= 1 << f;
+    }
+    if (!faces) return;
     // forward boundary state from S'_n(I)
     C f[Q];
 #pragma unroll
@@ -1021,36 +1034,66 @@ __global__ __launch_bounds__(256) void k_adjoint_ghost(AdjArgs a) {
     }
     const C uI[3] = {jx / r, jy / r, jz / r};
     const C cs = (C)CS;
-    const C sg = plus ? C(1) : C(-1);
-    const C W = uI[axis] + sg * cs * (r - C(1));
-    const C rB = C(1) + sg * W / (C(2) * cs);
-    C uB[3] = {uI[0], uI[1], uI[2]};
-    uB[axis] = W / C(2);
-    // adjoints of (rho_B, u_B) and (rho_I, u_I) from the feq terms
-    C aB = 0, vB[3] = {0, 0, 0}, aI = 0, vI[3] = {0, 0, 0};
+    C acc[Q];
 #pragma unroll
-    for (int q = 1; q < Q; ++q) {
-        if (gl[q] == C(0)) continue;
-        C dr, d0, d1, d2;
-        feq_partials<C>(q, rB, uB[0], uB[1], uB[2], dr, d0, d1, d2);
-        aB += gl[q] * dr; vB[0] += gl[q] * d0; vB[1] += gl[q] * d1; vB[2] += gl[q] * d2;
-        feq_partials<C>(q, r, uI[0], uI[1], uI[2], dr, d0, d1, d2);
-        aI += gl[q] * dr; vI[0] += gl[q] * d0; vI[1] += gl[q] * d1; vI[2] += gl[q] * d2;
+    for (int p = 0; p < Q; ++p) acc[p] = C(0);
+    C dM0 = C(0), dJ[3] = {C(0), C(0), C(0)};
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) {
+        if (!((faces >> fc) & 1)) continue;
+        const int axis = fc >> 1;
+        const bool plus = fc & 1;
+        const int hi = i + (axis == 0 ? (plus ? 1 : -1) : 0);
+        const int hj = j + (axis == 1 ? (plus ? 1 : -1) : 0);
+        const int hk = k + (axis == 2 ? (plus ? 1 : -1) : 0);
+        const long long h = cell_index(g, hi, hj, hk);
+        // adjoint of the pulled ghost slots (target inside the interior), cleared once read
+        C gl[Q];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            gl[q] = C(0);
+            const int ti = hi + EX[q], tj = hj + EY[q], tk = hk + EZ[q];
+            if (q == 0 || ti < 1 || ti > nx || tj < 1 || tj > ny || tk < 1 || tk > nz) continue;
+            const long long idx = pop_index<MODE>(q, h, N);
+            gl[q] = lam[idx];
+            lam[idx] = C(0);
+            any = any || gl[q] != C(0);
+        }
+        if (!any) continue;
+        const C sg = plus ? C(1) : C(-1);
+        const C W = uI[axis] + sg * cs * (r - C(1));
+        const C rB = C(1) + sg * W / (C(2) * cs);
+        C uB[3] = {uI[0], uI[1], uI[2]};
+        uB[axis] = W / C(2);
+        C aB = 0, vB[3] = {0, 0, 0}, aI = 0, vI[3] = {0, 0, 0};
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            if (gl[q] == C(0)) continue;
+            C dr, d0, d1, d2;
+            feq_partials<C>(q, rB, uB[0], uB[1], uB[2], dr, d0, d1, d2);
+            aB += gl[q] * dr; vB[0] += gl[q] * d0; vB[1] += gl[q] * d1; vB[2] += gl[q] * d2;
+            feq_partials<C>(q, r, uI[0], uI[1], uI[2], dr, d0, d1, d2);
+            aI += gl[q] * dr; vI[0] += gl[q] * d0; vI[1] += gl[q] * d1; vI[2] += gl[q] * d2;
+        }
+        // through rho_B(rho_I, u_I,a), u_B,a(rho_I, u_I,a) and u_B,t = u_I,t
+        const C dRho = aB * C(0.5) + vB[axis] * (sg * cs * C(0.5)) - aI;
+        C dU[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dU[c] = vB[c] - vI[c];
+        dU[axis] = aB * (sg / (C(2) * cs)) + vB[axis] * C(0.5) - vI[axis];
+        // through u_I = j / rho_I
+        const C ir = C(1) / r;
+        dM0 += dRho - (dU[0] * uI[0] + dU[1] * uI[1] + dU[2] * uI[2]) * ir;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dJ[c] += dU[c] * ir;
+#pragma unroll
+        for (int p = 0; p < Q; ++p) acc[p] += gl[p];
     }
-    // through rho_B(rho_I, u_I,a), u_B,a(rho_I, u_I,a) and u_B,t = u_I,t
-    C dRho = aB * C(0.5) + vB[axis] * (sg * cs * C(0.5)) - aI;
-    C dU[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) dU[c] = vB[c] - vI[c];
-    dU[axis] = aB * (sg / (C(2) * cs)) + vB[axis] * C(0.5) - vI[axis];
-    // through u_I = j / rho_I
-    const C ir = C(1) / r;
-    const C dM0 = dRho - (dU[0] * uI[0] + dU[1] * uI[1] + dU[2] * uI[2]) * ir;
-    const C dJ[3] = {dU[0] * ir, dU[1] * ir, dU[2] * ir};
 #pragma unroll
     for (int p = 0; p < Q; ++p) {
         const long long idx = pop_index<MODE>(p, I, N);
-        lam[idx] += gl[p] + dM0 + (C)EX[p] * dJ[0] + (C)EY[p] * dJ[1] + (C)EZ[p] * dJ[2];
+        lam[idx] += acc[p] + dM0 + (C)EX[p] * dJ[0] + (C)EY[p] * dJ[1] + (C)EZ[p] * dJ[2];
     }
 }
 
@@ -1724,8 +1767,8 @@ hipError_t Launch<MODE>::adjoint(const AdjArgs &a, hipStream_t s) {
     if constexpr (MODE == M_B16 || MODE == M_F16) {
         return hipErrorNotSupported;
     } else {
-        const long long total = (long long)a.g.nx * a.g.ny * a.g.nz;
-        k_adjoint<MODE><<<blocks_for(total, 256), 256, 0, s>>>(a);
+        const long long tiles = (long long)((a.g.nx + 255) >> 8) * a.g.ny * a.g.nz;
+        k_adjoint<MODE><<<(unsigned)tiles, 256, 0, s>>>(a);
         return hipGetLastError();
     }
 }
@@ -1735,8 +1778,10 @@ hipError_t Launch<MODE>::adjoint_ghost(const AdjArgs &a, hipStream_t s) {
     if constexpr (MODE == M_B16 || MODE == M_F16) {
         return hipErrorNotSupported;
     } else {
-        const int axis = a.face >> 1;
-        const long long n = (long long)(axis == 0 ? a.g.ny : a.g.nx) * (axis == 2 ? a.g.ny : a.g.nz);
+        const GridDesc &g = a.g;
+        const long long nxs = g.nx > 1 ? 2 : 1, nys = g.ny > 1 ? 2 : 1, nzs = g.nz > 1 ? 2 : 1;
+        const long long ix = g.nx > 2 ? g.nx - 2 : 0, iy = g.ny > 2 ? g.ny - 2 : 0;
+        const long long n = nxs * g.ny * g.nz + nys * ix * g.nz + nzs * ix * iy;
         k_adjoint_ghost<MODE><<<blocks_for(n, 256), 256, 0, s>>>(a);
         return hipGetLastError();
     }

@@ -46,7 +46,7 @@ struct Launch {
     static hipError_t source_multi(const SourceMulti &m, long long max_entries, hipStream_t s);
     // Discrete adjoint of one two-array step (see AdjArgs).
     static hipError_t adjoint(const AdjArgs &a, hipStream_t s);
-    // Transposed NSCBC ghost fill of face a.face (see AdjArgs), after adjoint().
+    // Transposed NSCBC ghost fill of every NSCBC face (see AdjArgs), after adjoint().
     static hipError_t adjoint_ghost(const AdjArgs &a, hipStream_t s);
     // out[e] = sum_q w_q lam[q][cells[e]], e < n (FP64).
     static hipError_t source_dot(const void *lam, long long ncell, const long long *cells, int n, double *out,

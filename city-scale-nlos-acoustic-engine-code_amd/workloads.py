@@ -233,9 +233,10 @@ class CanyonWorkload:
     """Adjoint source localisation in an urban canyon: nx x ny x nz cells at dx = 1 m, two
     parallel rows of seeded buildings along x (footprint lengths uniform 10-40 m, depth
     uniform 15-30 m, heights lognormal(ln 20, 0.5) capped at nz - 20) on either side of a
-    street of `street` cells, rigid ground, 16-cell sponge on five faces and the reference's
-    open boundary behind it (NSCBC on x-, x+, y-, y+ and the top, 05:684-835, which the
-    adjoint transposes; no building touches those faces).  The source is a static point source in the
+    street of `street` cells, rigid ground, 16-cell sponge on five faces, open faces at the
+    rest state, or with nscbc=True the reference's open boundary behind the sponge (NSCBC on
+    x-, x+, y-, y+ and the top, 05:684-835, which the adjoint transposes; no building touches
+    those faces).  The source is a static point source in the
     street, spread by a Gaussian stencil of sigma 2 cells (radius 3), driven by a sampled
     Ricker waveform (peak 100 Hz) of n_samples samples; 16 probes at 1.5 m in the street."""
     nx: int = 256
@@ -246,13 +247,17 @@ class CanyonWorkload:
     n_samples: int = 3000
     n_probes: int = 16
     tau: float = 0.5 + 1e-2
+    nscbc: bool = False
 
     def config(self, precision="f32") -> nlos.Config:
-        return nlos.default_config(x_min=0.0, x_max=float(self.nx), y_min=0.0, y_max=float(self.ny),
-                                   z_min=0.0, z_max=float(self.nz), dx=1.0, c_phys=343.0, tau=self.tau,
-                                   sponge_width=16.0, sponge_strength=0.1, sky_z=1e9,
-                                   stream_pattern="two_array", precision=precision,
-                                   world_offset=(0.0, 0.0, 0.0))
+        cfg = nlos.default_config(x_min=0.0, x_max=float(self.nx), y_min=0.0, y_max=float(self.ny),
+                                  z_min=0.0, z_max=float(self.nz), dx=1.0, c_phys=343.0, tau=self.tau,
+                                  sponge_width=16.0, sponge_strength=0.1, sky_z=1e9,
+                                  stream_pattern="two_array", precision=precision,
+                                  world_offset=(0.0, 0.0, 0.0))
+        if not self.nscbc:
+            cfg.nscbc_faces = 0
+        return cfg
 
     def boxes(self) -> List[Box]:
         rng = np.random.Generator(np.random.PCG64(self.seed))
