@@ -126,6 +126,8 @@ struct StepArgs {
     void *fb[2];                         // x-face buffers (x-, x+) or null, see below
     const uint32_t *wall_bits;           // two-array sweep: bit q set when c - e_q is solid
     const void *xh;                      // x-halo records or null, see below
+    void *vel;                           // velocity sweeps (the adjoint's replay): u of f_in,
+                                         // [3][ncell] in the compute type
 };
 
 // x-face buffers.  The cells I = (1, j, k) and (nx, j, k) feed the NSCBC ghosts of the x
@@ -224,6 +226,7 @@ struct AdjArgs {
     int n_probe_cells;
     const double *dl_dp;                  // dL/d(rho-1) of this step per unique probe cell, or null
     int nscbc_faces;                      // faces whose halo holds NSCBC ghosts (F_XM ...)
+    const void *vel;                      // u of the step's pulled populations, [3][ncell]
 };
 
 // One step of the legacy family-A mode (legacy_a.hip).  f holds the populations in

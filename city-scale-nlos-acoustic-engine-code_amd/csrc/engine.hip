@@ -2490,15 +2490,19 @@ NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_
         for (long long s = 0; s < e->rec_steps; ++s)
             dl[(size_t)s * nu + e->probes[p].unique] += dl_dprobe[p * max_steps + e->rec_start + s];
     std::vector<DevBuf> store((size_t)every + 1);
-    DevBuf lam[2], dl_dev, src_cells, src_adds, lam_src;
+    DevBuf lam[2], dl_dev, src_cells, src_adds, lam_src, vel;
     auto release = [&]() {
         for (DevBuf &b : store) b.release();
         lam[0].release(); lam[1].release(); dl_dev.release(); src_cells.release(); src_adds.release();
         lam_src.release();
+        vel.release();
     };
     auto bail = [&](int rc) { release(); return rc; };
     for (DevBuf &b : store)
         if (b.reserve(pb) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "cannot allocate %d recompute states", every + 1));
+    // u of every step of a segment, recorded by the replay (3 values per cell and step)
+    const size_t vel_step = (size_t)e->g.ncell * 3 * e->c_bytes;
+    if (vel.reserve(vel_step * every) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "cannot allocate velocities"));
     if (lam[0].reserve(lam_n * e->c_bytes) != hipSuccess || lam[1].reserve(lam_n * e->c_bytes) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate adjoint arrays"));
     if (nu > 0) {
@@ -2562,11 +2566,12 @@ NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_
             st.n_probe_cells = 0;
             st.fb[0] = st.fb[1] = nullptr;
             st.xh = nullptr;
+            st.vel = static_cast<char *>(vel.p) + vel_step * k;
             rc = hip(by_mode(e->mode, [&](auto L) {
                 using LL = decltype(L);
                 hipError_t r = LL::source(sa, LAY_AB, e->stream);
                 if (r == hipSuccess) r = LL::ghost(ga, LAY_AB, e->stream);
-                if (r == hipSuccess) r = LL::step_ab(st, false, e->block, e->stream);
+                if (r == hipSuccess) r = LL::step_ab_vel(st, e->block, e->stream);
                 return r;
             }), "recompute");
             if (rc) return bail(rc);
@@ -2592,6 +2597,7 @@ NLOS_API int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_
             a.n_probe_cells = (int)nu;
             a.dl_dp = nu ? dl_dev.as<double>() + (size_t)(n - e->rec_start) * nu : nullptr;
             a.nscbc_faces = e->cfg.nscbc_faces;
+            a.vel = static_cast<const char *>(vel.p) + vel_step * k;
             const int cnt = (int)(off[k + 1] - off[k]);
             rc = hip(by_mode(e->mode, [&](auto L) {
                 using LL = decltype(L);

@@ -614,7 +614,7 @@ struct AbIn<M_B16> {
     __device__ __forceinline__ float get(int q) const { return q == 0 ? r0 : b16_dec(v[q], e[q]); }
 };
 
-template <int MODE, bool CAPTURE>
+template <int MODE, bool CAPTURE, bool VEL = false>
 __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in, typename Tr<MODE>::C sig,
                                         int i, int j, int k, long long cb, unsigned t) {
     using C = typename Tr<MODE>::C;
@@ -653,6 +653,23 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
             }
         }
 
+        if constexpr (VEL) {
+            // u = j / rho of the pulled populations, which is all the transposed collision
+            // needs (k_adjoint): recorded here so the adjoint does not pull the state again
+            C r = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                r += f[q];
+                jx += f[q] * (C)EX[q];
+                jy += f[q] * (C)EY[q];
+                jz += f[q] * (C)EZ[q];
+            }
+            if constexpr (MODE != M_F64) r += C(1);   // deviations: rho = 1 + sum g
+            C *u = static_cast<C *>(a.vel);
+            u[c] = jx / r;
+            u[N + c] = jy / r;
+            u[2 * N + c] = jz / r;
+        }
         double dval;   // rho - 1
         bool finite;
         if constexpr (MODE == M_F64) {
@@ -730,7 +747,7 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
 }
 
 // Tile b: (bi, bj) of the x-y plane at plane k.
-template <int MODE, bool CAPTURE>
+template <int MODE, bool CAPTURE, bool VEL = false>
 __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     using C = typename Tr<MODE>::C;
     using S = typename Tr<MODE>::S;
@@ -794,12 +811,12 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
     // sigma(min d) = max over axes of sigma(d_axis): the ramp is monotone (05:491-506).
     const C sig = fmax(glb(static_cast<const C *>(a.sig_x))[i],
                        fmax(glb(static_cast<const C *>(a.sig_y))[j], glb(static_cast<const C *>(a.sig_z))[k]));
-    ab_cell<MODE, CAPTURE>(a, in, sig, i, j, k, cb, t);
+    ab_cell<MODE, CAPTURE, VEL>(a, in, sig, i, j, k, cb, t);
 }
 
-template <int MODE, bool CAPTURE>
+template <int MODE, bool CAPTURE, bool VEL = false>
 __global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
-    step_ab_body<MODE, CAPTURE>(a, blockIdx.x);
+    step_ab_body<MODE, CAPTURE, VEL>(a, blockIdx.x);
 }
 
 // Sweep over the tiles of several scenes: workgroup (t, s) sweeps tile t of scene s
@@ -873,29 +890,9 @@ __global__ __launch_bounds__(256) void k_adjoint(AdjArgs a) {
         return;
     }
     const uint32_t sb = (m & MK_NEAR) ? a.wall_bits[c] : 0u;   // bit q: c - e_q is solid
-    // pulled populations (full values) and moments
-    C f[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        C v;
-        if ((sb >> q) & 1u) {
-            v = ld<MODE>(a.state, pop_index<MODE>(INV[q], c, N), a.inv_scale);
-            if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
-        } else {
-            v = ld<MODE>(a.state, pop_index<MODE>(q, c - nb_off(g, q), N), a.inv_scale);
-        }
-        if constexpr (MODE != M_F64) v += wq<C>(q);   // deviation storage
-        f[q] = v;
-    }
-    C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        rho += f[q];
-        jx += f[q] * (C)EX[q];
-        jy += f[q] * (C)EY[q];
-        jz += f[q] * (C)EZ[q];
-    }
-    const C ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    // u of the pulled populations, recorded by the replay's sweep (k_step_ab with VEL)
+    const C *vel = static_cast<const C *>(a.vel);
+    const C ux = vel[c], uy = vel[N + c], uz = vel[2 * N + c];
     const C u2 = ux * ux + uy * uy + uz * uz;
     const C sig = fmax(static_cast<const C *>(a.sig_x)[i],
                        fmax(static_cast<const C *>(a.sig_y)[j], static_cast<const C *>(a.sig_z)[k]));
@@ -1635,6 +1632,16 @@ hipError_t Launch<MODE>::step(const StepArgs &a, bool odd, bool capture, dim3 bl
             if (capture) k_step<MODE, false, true><<<grid, block, 0, s>>>(a);
             else k_step<MODE, false, false><<<grid, block, 0, s>>>(a);
         }
+        return hipGetLastError();
+    }
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::step_ab_vel(const StepArgs &a, dim3 block, hipStream_t s) {
+    if constexpr (MODE == M_B16 || MODE == M_F16) {
+        return hipErrorNotSupported;
+    } else {
+        k_step_ab<MODE, false, true><<<dim3(a.nblocks), block, 0, s>>>(a);
         return hipGetLastError();
     }
 }

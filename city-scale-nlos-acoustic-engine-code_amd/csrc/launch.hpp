@@ -15,6 +15,9 @@ struct Launch {
     static hipError_t step(const StepArgs &a, bool odd, bool capture, dim3 block, hipStream_t s);
     // Fused pull sweep of the two-array pattern (LAY_AB): a.pop -> a.pop_out.
     static hipError_t step_ab(const StepArgs &a, bool capture, dim3 block, hipStream_t s);
+    // The same sweep, also recording u of each fluid cell's pulled populations in a.vel
+    // (the adjoint's replay; F64 and F32).
+    static hipError_t step_ab_vel(const StepArgs &a, dim3 block, hipStream_t s);
     // Ghost layer of the open faces (NSCBC or rest state) written into the current layout.
     static hipError_t ghost(const GhostArgs &a, int lay, hipStream_t s);
     // Pull step of solid cells on open faces (their state feeds the NSCBC ghosts).

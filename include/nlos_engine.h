@@ -328,7 +328,8 @@ int nlos_timing_get(nlos_engine *e, nlos_timing *t);
  * nlos_backward then returns the gradient of a loss L over the recorded steps, given
  * dL/d(rho-1) for every probe and step ([probe][step], stride max_steps, global step
  * indices as in nlos_read_probes).  It recomputes each segment of `every` steps from its
- * checkpoint (every + 1 population arrays of device memory), runs the adjoint step by step
+ * checkpoint (every + 1 population arrays and 3 values per cell and step of device
+ * memory), runs the adjoint step by step
  * back to the segment start, and accumulates in FP64:
  *   d_position: dL/d(shift of the whole source path, start and end together), per metre;
  *   d_amp:      dL/d amp;
