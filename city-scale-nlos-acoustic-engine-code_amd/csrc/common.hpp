@@ -174,9 +174,20 @@ struct GhostArgs {
     const void *s_old;                    // tracked state of solid cells on open faces
     const int *face_map[6];               // per NSCBC face: plane index -> state slot or -1
     float scale, inv_scale;
-    long long n_halo;
+    long long n_halo;                     // ghost_items(g, skip_zm)
     void *xh;                             // x-halo records or null (see StepArgs)
+    int skip_zm;                          // the k = 0 layer is solid ground: not enumerated
 };
+
+// Lanes per halo row of the ghost kernel: the row's cells i in [1, nx], 64-cell aligned.
+__host__ __device__ __forceinline__ long long ghost_row_lanes(int nx) { return ((long long)nx + 63) / 64 * 64; }
+// Work items of the ghost kernel: the rows of the z layers (k = nz + 1, and k = 0 unless
+// skip_zm) and the two y rows of every interior k, at ghost_row_lanes each plus their two
+// end cells (i = 0, nx + 1), then one item per x-halo cell of the interior rows.
+__host__ __device__ __forceinline__ long long ghost_items(const GridDesc &g, int skip_zm) {
+    const long long rows = (long long)(2 - (skip_zm ? 1 : 0)) * (g.ny + 2) + 2LL * g.nz;
+    return rows * (ghost_row_lanes(g.nx) + 2) + 2LL * g.ny * g.nz;
+}
 
 struct SolidArgs {
     void *pop;
@@ -251,7 +262,7 @@ struct SourceMulti {
     const SourceArgs *scenes;
     const long long *ctr, *ctr_base;
     const long long *step_off;
-    const int *entry_scene;
+    const int *entry_scene;               // scene of each entry (null: all of scene 0)
     const long long *cells;
     const double *adds;
 };

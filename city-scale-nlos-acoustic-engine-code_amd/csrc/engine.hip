@@ -269,6 +269,23 @@ struct nlos_engine {
     std::vector<long long> sched_cells;
     std::vector<double> sched_adds;
 
+    // nlos_step of the two-array pattern: every launch of a step finds the step (its source
+    // entries, its probe row) through a device counter, so two steps are captured once as a
+    // HIP graph and replayed while nothing they read moves (graph_key)
+    DevBuf g_ctr;                     // [0] the step, [1] first step of the staged schedule
+    DevBuf g_src;                     // SourceArgs of the steps that read pop and pop2
+    DevBuf sched_off_d;               // per-step entry offsets of the staged schedule
+    long long h_ctr[2] = {0, 0};      // host staging of g_ctr
+    SourceArgs h_src[2]{};            // host staging of g_src
+    hipGraph_t step_graph = nullptr;
+    hipGraphExec_t step_exec = nullptr;
+    std::vector<unsigned char> graph_key;   // launch arguments the graph was captured with
+    bool step_graph_on = true;        // NLOS_STEP_GRAPH=0: launch every step directly
+    // the tracked boundary solids of a step run on side_stream beside its sweep (fork after
+    // the ghost layer, join before the next step)
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     // legacy family-A mode (legacy_a.hip)
     int family = NLOS_FAMILY_B;
     DevBuf sigma_a;                   // sponge per interior cell (FP64)
@@ -643,10 +660,10 @@ int build_line_cells(nlos_engine *e);
 // swaps their two state buffers; phase 1 is the state after one such step.
 void *pop_at(const nlos_engine *e, int phase) { return ((e->cur ^ phase) & 1) ? e->pop2.p : e->pop.p; }
 void *solid_state_at(const nlos_engine *e, int phase) { return (phase & 1) ? e->s_b.p : e->s_a.p; }
-long long halo_cells(const GridDesc &g) {
-    const long long nx = g.nx, ny = g.ny, nz = g.nz;
-    return 2 * (nx + 2) * (ny + 2) + 2 * (nx + 2) * nz + 2 * ny * nz;
-}
+// The ground layer k = 0 is solid halo (05:520-525) unless the ground is open; the ghost
+// kernel then has nothing to write there.
+int skip_zm(const nlos_engine *e) { return e->cfg.ground && e->family == NLOS_FAMILY_B ? 1 : 0; }
+long long halo_cells(const nlos_engine *e) { return ghost_items(e->g, skip_zm(e)); }
 
 SourceArgs source_args(const nlos_engine *e, int phase) {
     SourceArgs a{};
@@ -673,7 +690,8 @@ GhostArgs ghost_args(const nlos_engine *e, int phase) {
     for (int f = 0; f < 6; ++f) a.face_map[f] = e->face_map[f].as<int>();
     a.scale = e->scale;
     a.inv_scale = e->inv_scale;
-    a.n_halo = halo_cells(e->g);
+    a.n_halo = halo_cells(e);
+    a.skip_zm = skip_zm(e);
     a.xh = e->xh.p;
     return a;
 }
@@ -760,16 +778,23 @@ int launch_halos(nlos_engine *e) {
     return NLOS_OK;
 }
 
-int launch_sweep(nlos_engine *e, bool capture) {
-    if (capture) {
-        // the field buffer may still be on its way to the host for a dump
-        if (e->dump_pending) HIP_TRY(hipStreamWaitEvent(e->stream, e->dump_copied, 0));
-        if (!e->field_alloc) {
-            HIP_TRY(e->field.reserve((size_t)e->n_interior() * 4));
-            e->field_alloc = true;
-        }
-        HIP_TRY(launch_fill_f32(e->field.as<float>(), e->n_interior(), 0.0f, e->stream));
+int launch_sweep_args(nlos_engine *e, const StepArgs &a, bool capture);
+
+// The field buffer of a capture sweep, cleared (solid cells keep 0).
+int prepare_capture(nlos_engine *e) {
+    // the field buffer may still be on its way to the host for a dump
+    if (e->dump_pending) HIP_TRY(hipStreamWaitEvent(e->stream, e->dump_copied, 0));
+    if (!e->field_alloc) {
+        HIP_TRY(e->field.reserve((size_t)e->n_interior() * 4));
+        e->field_alloc = true;
     }
+    HIP_TRY(launch_fill_f32(e->field.as<float>(), e->n_interior(), 0.0f, e->stream));
+    return NLOS_OK;
+}
+
+int launch_sweep(nlos_engine *e, bool capture) {
+    if (capture)
+        if (int rc = prepare_capture(e)) return rc;
     StepArgs a = step_args(e, 0, e->block);
     a.probe_row = a.n_probe_cells ? e->hist.as<double>() + (size_t)e->step * a.n_probe_cells : nullptr;
     a.field = capture ? e->field.as<float>() : nullptr;
@@ -781,6 +806,18 @@ int launch_sweep(nlos_engine *e, bool capture) {
                                             e->stream);
         }));
     e->walls_valid = true;
+    if (int rc = launch_sweep_args(e, a, capture)) return rc;
+    if (two) e->cur ^= 1;
+    else e->lay = odd ? LAY_Q : LAY_P;
+    if (capture) e->field_valid = true;
+    ++e->step;
+    return NLOS_OK;
+}
+
+// The sweep kernel with its timing events, if on; no change of host state.
+int launch_sweep_args(nlos_engine *e, const StepArgs &a, bool capture) {
+    const bool two = e->pattern == NLOS_PATTERN_TWO_ARRAY;
+    const bool odd = e->lay == LAY_P;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (e->timing) {
         if (e->ev_used + 2 > e->ev.size()) {
@@ -801,10 +838,6 @@ int launch_sweep(nlos_engine *e, bool capture) {
         HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::step(a, odd, capture, e->block, e->stream); }));
     }
     if (e->timing) HIP_TRY(hipEventRecord(t1, e->stream));
-    if (two) e->cur ^= 1;
-    else e->lay = odd ? LAY_Q : LAY_P;
-    if (capture) e->field_valid = true;
-    ++e->step;
     return NLOS_OK;
 }
 
@@ -852,6 +885,176 @@ int launch_a(nlos_engine *e, bool capture) {
     if (e->timing) HIP_TRY(hipEventRecord(t1, e->stream));
     if (capture) e->field_valid = true;
     ++e->step;
+    return NLOS_OK;
+}
+
+// ---- nlos_step of the two-array pattern: counter-indexed launches and the step graph ----
+long long stencil_bound(const nlos_engine *e);
+
+SourceMulti step_source(const nlos_engine *e, int phase) {
+    SourceMulti m{};
+    m.scenes = e->g_src.as<SourceArgs>() + ((e->cur ^ phase) & 1);   // g_src[k]: array k (pop, pop2)
+    m.ctr = e->g_ctr.as<long long>();
+    m.ctr_base = e->g_ctr.as<long long>() + 1;
+    m.step_off = e->sched_off_d.as<long long>();
+    m.entry_scene = nullptr;
+    m.cells = e->src_cells.as<long long>();
+    m.adds = e->src_adds.as<double>();
+    return m;
+}
+
+// Sweep arguments of the state `phase` steps ahead whose probe row is hist + step * n,
+// step read from the device counter.
+StepArgs step_args_ctr(const nlos_engine *e, int phase) {
+    StepArgs a = step_args(e, phase, e->block);
+    a.step_ctr = e->g_ctr.as<long long>();
+    a.hist = e->hist.as<double>();
+    if (!a.hist) a.n_probe_cells = 0;
+    return a;
+}
+
+// One step `phase` steps ahead of the host state: source entries of the step the device
+// counter names, ghost layer, tracked boundary solids, sweep, counter increment.  Leaves
+// the host state as it is (advance_steps moves it).
+int launch_step_ctr(nlos_engine *e, int phase, bool capture) {
+    const long long me = stencil_bound(e);
+    if (me > 0) {
+        const SourceMulti m = step_source(e, phase);
+        HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::source_multi(m, me, e->stream); }));
+    }
+    const GhostArgs ga = ghost_args(e, phase);
+    HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::ghost(ga, e->lay, e->stream); }));
+    // The tracked solids pull from this step's state and ghosts and write only their own
+    // next state, which nothing reads before the next step's ghost layer: they run beside
+    // the sweep (at 1024x1024x256 they took 21 us of every step in line).
+    const bool side = e->n_solid > 0;
+    if (side) {
+        const SolidArgs sa = solid_args(e, phase);
+        HIP_TRY(hipEventRecord(e->ev_fork, e->stream));
+        HIP_TRY(hipStreamWaitEvent(e->side_stream, e->ev_fork, 0));
+        HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::solid(sa, e->side_stream); }));
+        HIP_TRY(hipEventRecord(e->ev_join, e->side_stream));
+    }
+    if (capture)
+        if (int rc = prepare_capture(e)) return rc;
+    StepArgs a = step_args_ctr(e, phase);
+    a.field = capture ? e->field.as<float>() : nullptr;
+    if (int rc = launch_sweep_args(e, a, capture)) return rc;
+    HIP_TRY(launch_count(e->g_ctr.as<long long>(), 1, e->stream));
+    if (side) HIP_TRY(hipStreamWaitEvent(e->stream, e->ev_join, 0));
+    return NLOS_OK;
+}
+
+// Host state after n steps of launch_step_ctr: the arrays and the tracked-solid buffers
+// swap once per step.
+void advance_steps(nlos_engine *e, long long n, bool captured) {
+    if (n & 1) {
+        e->cur ^= 1;
+        if (e->n_solid > 0) std::swap(e->s_a, e->s_b);
+    }
+    e->step += n;
+    if (captured) e->field_valid = true;
+}
+
+template <class T>
+void key_append(std::vector<unsigned char> &k, const T &v) {
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+    k.insert(k.end(), p, p + sizeof(T));
+}
+
+// Everything the two captured steps bake into their launches.
+std::vector<unsigned char> step_graph_key(const nlos_engine *e) {
+    std::vector<unsigned char> k;
+    for (int ph = 0; ph < 2; ++ph) {
+        key_append(k, step_args_ctr(e, ph));
+        key_append(k, ghost_args(e, ph));
+        key_append(k, solid_args(e, ph));
+        key_append(k, step_source(e, ph));
+    }
+    key_append(k, stencil_bound(e));
+    key_append(k, e->n_solid);
+    key_append(k, e->lay);
+    key_append(k, e->block);
+    return k;
+}
+
+int drop_step_graph(nlos_engine *e) {
+    if (e->step_exec) HIP_TRY(hipGraphExecDestroy(e->step_exec));
+    if (e->step_graph) HIP_TRY(hipGraphDestroy(e->step_graph));
+    e->step_exec = nullptr;
+    e->step_graph = nullptr;
+    e->graph_key.clear();
+    return NLOS_OK;
+}
+
+// The graph of two steps (the current state and the one after it), captured again when
+// any argument it bakes in has moved.
+int ensure_step_graph(nlos_engine *e) {
+    std::vector<unsigned char> key = step_graph_key(e);
+    if (e->step_exec && key == e->graph_key) return NLOS_OK;
+    if (int rc = drop_step_graph(e)) return rc;
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step_ctr(e, 0, false);
+    if (!rc) rc = launch_step_ctr(e, 1, false);
+    hipGraph_t g = nullptr;
+    const hipError_t he = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (he != hipSuccess) return fail(NLOS_ERR_HIP, "step graph capture: %s", hipGetErrorString(he));
+    e->step_graph = g;
+    HIP_TRY(hipGraphInstantiate(&e->step_exec, e->step_graph, nullptr, nullptr, 0));
+    e->graph_key = std::move(key);
+    return NLOS_OK;
+}
+
+// Stages the schedule of steps e->step .. e->step + m - 1 (already in e->sched_*) for
+// counter-indexed launches: entries, per-step offsets, source arguments, counter.
+int stage_counter_schedule(nlos_engine *e) {
+    if (int rc = upload_schedule(e)) return rc;
+    const size_t nb = e->sched_off.size() * 8;
+    if (e->sched_off_d.bytes < nb) {
+        e->sched_off_d.release();
+        HIP_TRY(e->sched_off_d.reserve(std::max<size_t>(nb, 4097 * 8)));
+    }
+    if (!e->g_ctr.p) HIP_TRY(e->g_ctr.reserve(2 * sizeof(long long)));
+    if (e->n_solid > 0 && !e->side_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&e->side_stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+    }
+    if (!e->g_src.p) HIP_TRY(e->g_src.reserve(2 * sizeof(SourceArgs)));
+    e->h_ctr[0] = e->step;
+    e->h_ctr[1] = e->step;
+    // per population array (pop, pop2), whichever is current
+    e->h_src[0] = source_args(e, e->cur & 1);
+    e->h_src[1] = source_args(e, (e->cur ^ 1) & 1);
+    HIP_TRY(hipMemcpyAsync(e->sched_off_d.p, e->sched_off.data(), nb, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->g_ctr.p, e->h_ctr, sizeof(e->h_ctr), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->g_src.p, e->h_src, sizeof(e->h_src), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));   // the staging above is host memory of the engine
+    return sync_face_buffers(e);
+}
+
+// Steps 0 .. m-1 of the staged schedule; the last one captures the field when asked.  Pairs
+// of steps replay the step graph unless timing is on (its events sit around every sweep)
+// or NLOS_STEP_GRAPH=0; the rest launch directly, with the same kernels and arguments.
+int run_counter_steps(nlos_engine *e, long long m, bool capture_last) {
+    long long s = 0;
+    const long long direct_tail = capture_last ? 1 : 0;
+    if (e->step_graph_on && !e->timing && m - direct_tail >= 2) {
+        if (int rc = ensure_step_graph(e)) return rc;
+        for (; s + 2 <= m - direct_tail; s += 2) {
+            HIP_TRY(hipGraphLaunch(e->step_exec, e->stream));
+            advance_steps(e, 2, false);
+        }
+    }
+    for (; s < m; ++s) {
+        const bool cap = capture_last && s == m - 1;
+        if (int rc = launch_step_ctr(e, 0, cap)) return rc;
+        advance_steps(e, 1, cap);
+    }
     return NLOS_OK;
 }
 
@@ -1028,6 +1231,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         if (e->xh.reserve(xb) != hipSuccess) return bail(fail(NLOS_ERR_NOMEM, "cannot allocate x-halo records"));
         if (hipMemset(e->xh.p, 0, xb) != hipSuccess) return bail(fail(NLOS_ERR_HIP, "memset x-halo records"));
     }
+    if (const char *x = std::getenv("NLOS_STEP_GRAPH")) e->step_graph_on = std::atoi(x) != 0;
     if (e->mask.reserve((size_t)e->g.ncell) != hipSuccess || e->flag.reserve(sizeof(int)) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
     for (int f = 0; f < 2; ++f)
@@ -1078,6 +1282,15 @@ NLOS_API int nlos_destroy(nlos_engine *e) {
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : e->ck) b.release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
+    if (e->step_exec) (void)hipGraphExecDestroy(e->step_exec);
+    if (e->step_graph) (void)hipGraphDestroy(e->step_graph);
+    if (e->side_stream) (void)hipStreamSynchronize(e->side_stream);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    if (e->side_stream) (void)hipStreamDestroy(e->side_stream);
+    e->g_ctr.release();
+    e->g_src.release();
+    e->sched_off_d.release();
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return NLOS_OK;
@@ -1450,12 +1663,17 @@ NLOS_API int nlos_step(nlos_engine *e, int64_t n, int32_t flags) {
             schedule_step(e, (double)(e->step + s) * e->dt);      // t = s*dt (05:1079)
             e->sched_off.push_back((long long)e->sched_cells.size());
         }
-        if ((rc = upload_schedule(e))) return rc;
-        for (long long s = 0; s < m; ++s) {
-            if ((rc = launch_source(e, e->sched_off[s], e->sched_off[s + 1] - e->sched_off[s]))) return rc;
-            if ((rc = launch_halos(e))) return rc;
-            const bool cap = (flags & NLOS_STEP_CAPTURE_FIELD) && base + s == n - 1;
-            if ((rc = launch_sweep(e, cap))) return rc;
+        if (e->pattern == NLOS_PATTERN_TWO_ARRAY) {
+            if ((rc = stage_counter_schedule(e))) return rc;
+            if ((rc = run_counter_steps(e, m, (flags & NLOS_STEP_CAPTURE_FIELD) && base + m == n))) return rc;
+        } else {
+            if ((rc = upload_schedule(e))) return rc;
+            for (long long s = 0; s < m; ++s) {
+                if ((rc = launch_source(e, e->sched_off[s], e->sched_off[s + 1] - e->sched_off[s]))) return rc;
+                if ((rc = launch_halos(e))) return rc;
+                const bool cap = (flags & NLOS_STEP_CAPTURE_FIELD) && base + s == n - 1;
+                if ((rc = launch_sweep(e, cap))) return rc;
+            }
         }
         if (e->timing && (rc = collect_timing(e))) return rc;
     }
@@ -2090,7 +2308,7 @@ int batch_prepare(nlos_batch *b, bool &moved) {
         }
         const int scene = (int)tiles.size();
         tiles.push_back(st[0].back().nblocks);
-        for (long long h = 0; h < halo_cells(e->g); h += 256) halo.push_back(make_int2(scene, (int)h));
+        for (long long h = 0; h < halo_cells(e); h += 256) halo.push_back(make_int2(scene, (int)h));
         for (int q = 0; q < e->n_solid; q += 256) solid.push_back(make_int2(scene, q));
         b->max_entries += stencil_bound(e);
     }

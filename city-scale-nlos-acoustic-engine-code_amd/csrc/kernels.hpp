@@ -1143,27 +1143,41 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
     const GridDesc &g = a.g;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     int i, j, k;
-    // Halo enumeration: the two z layers, then the two y rows of every interior k, then
-    // the two x columns of every interior (j,k).
-    const long long zpl = (long long)(nx + 2) * (ny + 2);
-    const long long yrow = (long long)(nx + 2) * nz;
-    if (t < 2 * zpl) {
-        k = t < zpl ? 0 : nz + 1;
-        const long long r = t < zpl ? t : t - zpl;
-        j = (int)(r / (nx + 2));
-        i = (int)(r % (nx + 2));
-    } else if ((t -= 2 * zpl) < 2 * yrow) {
-        j = t < yrow ? 0 : ny + 1;
-        const long long r = t < yrow ? t : t - yrow;
-        k = (int)(r / (nx + 2)) + 1;
-        i = (int)(r % (nx + 2));
+    // Halo enumeration: the z layers (k = 0 unless it is solid ground, then k = nz + 1) and the
+    // two y rows of every interior k, their cells i in [1, nx] by rows of W = roundup(nx, 64)
+    // lanes, so that every wave covers one 64-cell-aligned run (interior cells start
+    // 64-aligned at i = 1) and its loads and stores fill whole lines; then the same rows'
+    // cells i = 0 and i = nx + 1; then the two x columns of every interior (j, k).
+    const long long W = ghost_row_lanes(nx);
+    const int k0 = a.skip_zm ? 1 : 0;
+    const long long zrows = (long long)(2 - k0) * (ny + 2);
+    const long long rows = zrows + 2LL * nz;
+    long long row;
+    if (t < rows * W) {
+        row = t / W;
+        i = (int)(t - row * W) + 1;
+        if (i > nx) return;
+    } else if ((t -= rows * W) < 2 * rows) {
+        row = t >> 1;
+        i = (t & 1) ? nx + 1 : 0;
     } else {
-        t -= 2 * yrow;
+        t -= 2 * rows;
         const long long xcol = (long long)ny * nz;
         i = t < xcol ? 0 : nx + 1;
         const long long r = t < xcol ? t : t - xcol;
         k = (int)(r / ny) + 1;
         j = (int)(r % ny) + 1;
+        row = -1;
+    }
+    if (row >= 0) {
+        if (row < zrows) {
+            k = ((int)(row / (ny + 2)) + k0) == 0 ? 0 : nz + 1;
+            j = (int)(row % (ny + 2));
+        } else {
+            const long long r = row - zrows;
+            j = (r & 1) ? ny + 1 : 0;
+            k = (int)(r >> 1) + 1;
+        }
     }
     const long long N = g.ncell;
     const long long h = cell_index(g, i, j, k);
@@ -1369,20 +1383,16 @@ __device__ __forceinline__ void solid_body(const SolidArgs &a, int n) {
     const long long N = a.g.ncell;
     const C *so = static_cast<const C *>(a.s_old);
     C *sn = static_cast<C *>(a.s_new);
+    // the cell's (i, j, k), once: its source cell of slot q is (i, j, k) - e_q
+    const long long r = c - a.g.origin;
+    const int ci = (int)(r % a.g.px), cj = (int)((r / a.g.px) % a.g.py), ck = (int)(r / a.g.plane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const long long src = c - nb_off(a.g, q);
         C v;
         // an x-halo source keeps this slot in the records (see StepArgs::xh)
-        bool xsrc = false;
-        int sj = 0, sk = 0;
-        if (a.xh && EX[q] != 0) {
-            const long long r = src - a.g.origin;
-            const int si = (int)(r % a.g.px);
-            sj = (int)((r / a.g.px) % a.g.py);
-            sk = (int)(r / a.g.plane);
-            xsrc = EX[q] > 0 ? si == 0 : si == a.g.nx + 1;
-        }
+        const bool xsrc = a.xh && EX[q] != 0 && (EX[q] > 0 ? ci - 1 == 0 : ci + 1 == a.g.nx + 1);
+        const int sj = cj - EY[q], sk = ck - EZ[q];
         if (a.mask[src] & MK_SOLID) {
             v = so[(long long)n * Q + INV[q]];
             if (a.alpha != 1.0) v = reflect<MODE>(q, v, a.alpha);
@@ -1511,7 +1521,7 @@ __global__ __launch_bounds__(256) void k_source_multi(SourceMulti m) {
     constexpr int PER = MODE == M_B16 ? 1 : Q;   // threads per entry
     if (t >= (e1 - e0) * PER) return;
     const long long e = e0 + t / PER;
-    SourceArgs a = m.scenes[m.entry_scene[e]];
+    SourceArgs a = m.scenes[m.entry_scene ? m.entry_scene[e] : 0];   // null: one scene
     a.cells = m.cells + e;
     a.adds = m.adds + e;
     a.n = 1;
