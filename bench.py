@@ -570,9 +570,11 @@ def bench_legacy04(args) -> None:
     cells = nx * ny * nz
     n_solid = int(solid.sum())
     mlups = cells * steps / wall / 1e6
-    # per fluid cell and step: collide 19 loads + 19 stores, stream 19 loads + 19 stores,
-    # FP64, plus the mask byte and the sponge value; solid cells 1 B
-    bpu = 76 * 8 + 1 + 8
+    # per fluid cell and step, FP64, plus the mask byte and the sponge value (solid cells 1 B):
+    # the fused pass pulls 19 post-collision values and stores the next step's 19; the two
+    # passes of NLOS_FUSED_A=0 load and store 19 each in the collide and the stream pass
+    fused = os.environ.get("NLOS_FUSED_A", "1") != "0"
+    bpu = (38 if fused else 76) * 8 + 1 + 8
     alg = (cells - n_solid) * bpu + n_solid
     out = {"metric": "MLUPS, legacy family-A step (04_hpc_scalability geometry 208^3, FP64)",
            "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -581,7 +583,7 @@ def bench_legacy04(args) -> None:
            "config": {"workload": f"04 benchmark: {nx}x{ny}x{nz} cells, cylinder r=4, radial sponge, "
                                   "z periodic, 04's line source, 100 warm-up + timed steps"},
            "probe_max_abs": float(np.abs(probe).max()),
-           "sweep_ms": round(tm.avg_ms, 4),
+           "sweep_ms": round(tm.avg_ms, 4), "step_form": "one fused pass" if fused else "collide + stream passes",
            "roofline": {"bound": "hbm", "bytes_per_update": bpu,
                         "achieved": round(alg / (wall / steps) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(alg / (wall / steps) / 1e9 / HBM_PEAK_GBS, 4)}}

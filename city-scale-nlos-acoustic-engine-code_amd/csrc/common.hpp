@@ -256,6 +256,32 @@ struct AStepArgs {
     int *flag;                            // bit 0: non-finite density seen
 };
 
+// One family-A step in one pass (legacy_a.hip, k_a_fused).  The stream of the step pulls
+// from `post`, the post-collision values of the step's start state (every fluid cell, halo
+// cells included), and bounces from `fb`, that start state after its line-source add, read
+// only at cells next to a wall.  The last step of a run writes the new state to f_out;
+// every other step applies the next step's line-source add to the new state and collides it
+// right away: post_out receives the next step's post-collision values (planes nz and 1 also
+// into their periodic z-halo planes 0 and nz + 1), fb_out the next bounce sources.
+constexpr uint8_t MK_LINE = 8;            // family A: fluid cell of the line source (line_factor)
+struct AFusedArgs {
+    const double *post;
+    const double *fb;
+    double *f_out;                        // last step (else null)
+    double *post_out, *fb_out;            // other steps
+    const uint8_t *mask;
+    const double *sigma;                  // sponge per interior cell, x fastest
+    const double *line_factor;            // exp(-ln2 d^2 / b) per interior cell, x fastest (MK_LINE)
+    double amp, sin_t, dt;                // line source of the next step: f += (w (amp s) sin_t) dt
+    GridDesc g;
+    double tau;
+    const long long *probe_cells;
+    int n_probe_cells;
+    double *probe_row;
+    double *field;
+    int *flag;
+};
+
 // Source entries of several scenes over a run of steps, read at the step the device counter
 // names: entries step_off[t] .. step_off[t+1] of step t = *ctr - *ctr_base.
 struct SourceMulti {

@@ -291,6 +291,10 @@ struct nlos_engine {
     DevBuf sigma_a;                   // sponge per interior cell (FP64)
     nlos_line_source line{};          // 04's line source (amp 0: none)
     DevBuf line_cells, line_spatial;  // its fluid cells and exp(-ln2 d2 / b) per cell
+    DevBuf line_factor;               // the same factors per interior cell, x fastest (0 elsewhere)
+    DevBuf post_b, fb_b;              // fused family-A step: second post array, bounce sources
+    bool post_b_ready = false;        // post_b holds the halo's post-collision values
+    bool fuse_a = true;               // NLOS_FUSED_A=0: two passes per family-A step
     long long n_line = 0;
     DevBuf field_a;                   // rho per interior cell of the last capture step (FP64)
     std::vector<double> rho0;         // density set before the first step (t = 0 probe line)
@@ -888,6 +892,81 @@ int launch_a(nlos_engine *e, bool capture) {
     return NLOS_OK;
 }
 
+// n family-A steps, each one pass of k_a_fused (see AFusedArgs), after a prime that applies
+// the first step's line-source add and halo to the state and collides it (the first half of
+// launch_a).  Post-collision values alternate between pop2 and post_b; the bounce sources of
+// cells next to a wall between pop and fb_b; the last pass writes the new state into pop.
+int run_a_fused(nlos_engine *e, long long n, bool capture_last) {
+    const GridDesc &g = e->g;
+    const size_t bytes = (size_t)g.ncell * Q * 8;
+    HIP_TRY(e->post_b.reserve(bytes));
+    HIP_TRY(e->fb_b.reserve(bytes));
+    double *pop = static_cast<double *>(e->pop.p);
+    auto sin_at = [&](long long step) {
+        // t = s * dt (04:371, 383), the timed loop's s counted from step0
+        return std::sin(e->line.omega * ((double)(step - e->line.step0) * e->line.dt));
+    };
+    HIP_TRY(launch_a_mark_line(e->mask.as<uint8_t>(), g, e->line_cells.as<long long>(), e->n_line, e->stream));
+    if (e->n_line > 0)
+        HIP_TRY(launch_a_line(pop, g, e->line_cells.as<long long>(), e->line_spatial.as<double>(), e->n_line,
+                              e->line.amp, sin_at(e->step), e->line.dt, e->stream));
+    HIP_TRY(launch_a_halo(pop, g, e->stream));
+    double *post[2] = {static_cast<double *>(e->pop2.p), e->post_b.as<double>()};
+    double *fbs[2] = {pop, e->fb_b.as<double>()};
+    HIP_TRY(launch_a_collide(pop, post[0], e->mask.as<uint8_t>(), g, e->cfg.tau, e->stream));
+    if (n > 1 && !e->post_b_ready) {
+        // the halo cells' post-collision values, which no pass rewrites
+        HIP_TRY(launch_a_collide(pop, post[1], e->mask.as<uint8_t>(), g, e->cfg.tau, e->stream));
+        e->post_b_ready = true;
+    }
+    if (capture_last) {
+        if (e->dump_pending) HIP_TRY(hipStreamWaitEvent(e->stream, e->dump_copied, 0));
+        HIP_TRY(e->field_a.reserve((size_t)e->n_interior() * 8));
+    }
+    const int nu = (int)e->probe_ucells.size();
+    for (long long s = 0; s < n; ++s) {
+        const bool last = s == n - 1;
+        AFusedArgs a{};
+        a.post = post[s & 1];
+        a.fb = fbs[s & 1];
+        a.f_out = last ? pop : nullptr;
+        a.post_out = last ? nullptr : post[(s + 1) & 1];
+        a.fb_out = last ? nullptr : fbs[(s + 1) & 1];
+        a.mask = e->mask.as<uint8_t>();
+        a.sigma = e->sigma_a.as<double>();
+        a.line_factor = e->line_factor.as<double>();
+        a.amp = e->line.amp;
+        a.sin_t = e->n_line > 0 ? sin_at(e->step + 1) : 0.0;
+        a.dt = e->line.dt;
+        a.g = g;
+        a.tau = e->cfg.tau;
+        a.n_probe_cells = nu;
+        a.probe_cells = nu ? e->probe_cells.as<long long>() : nullptr;
+        a.probe_row = nu ? e->hist.as<double>() + (size_t)e->step * nu : nullptr;
+        a.field = capture_last && last ? e->field_a.as<double>() : nullptr;
+        a.flag = e->flag.as<int>();
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (e->timing) {
+            if (e->ev_used + 2 > e->ev.size()) {
+                for (int q = 0; q < 64; ++q) {
+                    hipEvent_t x;
+                    HIP_TRY(hipEventCreate(&x));
+                    e->ev.push_back(x);
+                }
+            }
+            t0 = e->ev[e->ev_used];
+            t1 = e->ev[e->ev_used + 1];
+            e->ev_used += 2;
+            HIP_TRY(hipEventRecord(t0, e->stream));
+        }
+        HIP_TRY(launch_a_fused(a, e->stream));
+        if (e->timing) HIP_TRY(hipEventRecord(t1, e->stream));
+        ++e->step;
+    }
+    if (capture_last) e->field_valid = true;
+    return NLOS_OK;
+}
+
 // ---- nlos_step of the two-array pattern: counter-indexed launches and the step graph ----
 long long stencil_bound(const nlos_engine *e);
 
@@ -1232,6 +1311,7 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
         if (hipMemset(e->xh.p, 0, xb) != hipSuccess) return bail(fail(NLOS_ERR_HIP, "memset x-halo records"));
     }
     if (const char *x = std::getenv("NLOS_STEP_GRAPH")) e->step_graph_on = std::atoi(x) != 0;
+    if (const char *x = std::getenv("NLOS_FUSED_A")) e->fuse_a = std::atoi(x) != 0;
     if (e->mask.reserve((size_t)e->g.ncell) != hipSuccess || e->flag.reserve(sizeof(int)) != hipSuccess)
         return bail(fail(NLOS_ERR_NOMEM, "cannot allocate mask"));
     for (int f = 0; f < 2; ++f)
@@ -1278,7 +1358,8 @@ NLOS_API int nlos_destroy(nlos_engine *e) {
                       &e->src_adds, &e->face_map[0], &e->face_map[1], &e->face_map[2],
                       &e->face_map[3], &e->face_map[4], &e->face_map[5], &e->gh, &e->vox_tri,
                       &e->vox_bins, &e->vox_start, &e->rho_tmp, &e->fb[0], &e->fb[1], &e->wall_bits,
-                      &e->sigma_a, &e->field_a, &e->xh, &e->line_cells, &e->line_spatial};
+                      &e->sigma_a, &e->field_a, &e->xh, &e->line_cells, &e->line_spatial,
+                      &e->line_factor, &e->post_b, &e->fb_b};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : e->ck) b.release();
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
@@ -1482,7 +1563,9 @@ int build_line_cells(nlos_engine *e) {
     const nlos_line_source &s = e->line;
     std::vector<long long> cells;
     std::vector<double> spatial;
+    std::vector<double> factor;       // per interior cell, for the fused step (k_a_fused)
     if (s.amp != 0.0) {
+        factor.assign((size_t)e->n_interior(), 0.0);
         const nlos_config &c = e->cfg;
         const double ln2 = std::log(2.0);
         for (int k = 0; k < e->g.nz; ++k)
@@ -1495,6 +1578,7 @@ int build_line_cells(nlos_engine *e) {
                     if (e->solid_interior(i, j, k)) continue;
                     cells.push_back(e->cell(i + 1, j + 1, k + 1));
                     spatial.push_back(std::exp(-ln2 * d2 / s.b));
+                    factor[(size_t)i + (size_t)e->g.nx * ((size_t)j + (size_t)e->g.ny * k)] = spatial.back();
                 }
     }
     if (!cells.empty()) {
@@ -1502,6 +1586,8 @@ int build_line_cells(nlos_engine *e) {
         HIP_TRY(e->line_spatial.reserve(spatial.size() * 8));
         HIP_TRY(hipMemcpy(e->line_cells.p, cells.data(), cells.size() * 8, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(e->line_spatial.p, spatial.data(), spatial.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(e->line_factor.reserve(factor.size() * 8));
+        HIP_TRY(hipMemcpy(e->line_factor.p, factor.data(), factor.size() * 8, hipMemcpyHostToDevice));
     }
     e->n_line = (long long)cells.size();
     return NLOS_OK;
@@ -1648,8 +1734,12 @@ NLOS_API int nlos_step(nlos_engine *e, int64_t n, int32_t flags) {
     int rc = ensure_hist(e, e->step + n);
     if (rc) return rc;
     if (e->family == NLOS_FAMILY_A) {
-        for (long long s = 0; s < n; ++s)
-            if ((rc = launch_a(e, (flags & NLOS_STEP_CAPTURE_FIELD) && s == n - 1))) return rc;
+        if (e->fuse_a) {
+            if ((rc = run_a_fused(e, n, flags & NLOS_STEP_CAPTURE_FIELD))) return rc;
+        } else {
+            for (long long s = 0; s < n; ++s)
+                if ((rc = launch_a(e, (flags & NLOS_STEP_CAPTURE_FIELD) && s == n - 1))) return rc;
+        }
         if (e->timing && (rc = collect_timing(e))) return rc;
         return NLOS_OK;
     }

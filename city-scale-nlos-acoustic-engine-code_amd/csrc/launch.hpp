@@ -81,6 +81,12 @@ hipError_t launch_pull_pattern16(const void *in, void *out, const GridDesc &g, h
 // stream (01's step_pull).
 hipError_t launch_a_halo(double *f, const GridDesc &g, hipStream_t s);
 hipError_t launch_a_sweep(const AStepArgs &a, hipStream_t s);
+// The fused family-A step (see AFusedArgs), the post-collision values of every fluid cell of
+// a state, and the MK_LINE bits of the line source's cells.
+hipError_t launch_a_fused(const AFusedArgs &a, hipStream_t s);
+hipError_t launch_a_collide(const double *f, double *post, const uint8_t *mask, const GridDesc &g, double tau,
+                            hipStream_t s);
+hipError_t launch_a_mark_line(uint8_t *mask, const GridDesc &g, const long long *cells, long long n, hipStream_t s);
 // 04's line source: f_q(cells[e]) += (w_q * ((amp * spatial[e]) * sin_t)) * dt, e < n.
 hipError_t launch_a_line(double *f, const GridDesc &g, const long long *cells, const double *spatial, long long n,
                          double amp, double sin_t, double dt, hipStream_t s);

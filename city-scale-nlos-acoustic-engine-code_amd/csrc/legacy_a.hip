@@ -149,6 +149,123 @@ __global__ __launch_bounds__(256) void k_a_stream(AStepArgs a) {
     }
 }
 
+// The stream of k_a_stream and, for every step but the last of a run, the next step's
+// line-source add (k_a_line) and collision (k_a_collide) of the same cell, in one pass: the
+// post-collision values are pulled straight from the previous pass, so the state itself is
+// written once, at the end of the run (617 -> about 313 algorithmic bytes per update).  Every
+// value is formed by the same operations in the same order as in the separate kernels.
+__global__ __launch_bounds__(256) void k_a_fused(AFusedArgs a) {
+    const GridDesc &g = a.g;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.nx * g.ny * g.nz;
+    if (n >= total) return;
+    const int i = (int)(n % g.nx) + 1;
+    const int j = (int)((n / g.nx) % g.ny) + 1;
+    const int k = (int)(n / ((long long)g.nx * g.ny)) + 1;
+    const long long c = cell_index(g, i, j, k);
+    const uint8_t m = a.mask[c];
+    if (m & MK_SOLID) {
+        if (a.field) a.field[n] = 1.0;   // 01:367 writes rho = 1 for solid cells
+        return;
+    }
+    const double s = a.sigma[n];
+    const bool in_sponge = s > 1e-6;
+    double out[Q];
+    bool wall = false;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const long long src = c - off_a(g, q);
+        if (a.mask[src] & MK_SOLID) {
+            out[q] = a.fb[pop_index<M_F64>(INVA[q], c, g.ncell)];
+            wall = true;
+        } else {
+            const double p = a.post[pop_index<M_F64>(q, src, g.ncell)];
+            out[q] = in_sponge ? (1.0 - s) * p + s * W64[q] : p;   // feq(q, 1, 0, 0, 0) = w_q
+        }
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) sum += out[q];
+    if (!isfinite(sum)) atomicOr(a.flag, 1);
+    if (a.field) a.field[n] = sum;
+    if ((m & MK_PROBE) && a.probe_row) {
+        int lo = 0, hi = a.n_probe_cells - 1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const long long v = a.probe_cells[mid];
+            if (v == c) { a.probe_row[mid] = sum - 1.0; break; }
+            if (v < c) lo = mid + 1; else hi = mid - 1;
+        }
+    }
+    if (a.f_out) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(out[q], a.f_out + pop_index<M_F64>(q, c, g.ncell));
+        return;
+    }
+    // the next step: its line-source add (04:229-236), then its collision (01:315-343)
+    if (m & MK_LINE) {
+        const double v = a.amp * a.line_factor[n] * a.sin_t;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) out[q] += W64[q] * v * a.dt;
+    }
+    if (wall) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) a.fb_out[pop_index<M_F64>(q, c, g.ncell)] = out[q];
+    }
+    double sr = 0.0, sux = 0.0, suy = 0.0, suz = 0.0;
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+        sr += out[p];
+        sux += out[p] * (double)EXA[p];
+        suy += out[p] * (double)EYA[p];
+        suz += out[p] * (double)EZA[p];
+    }
+    if (sr == 0.0) sr = 1.0;
+    sux /= sr;
+    suy /= sr;
+    suz /= sr;
+    // z periodic (01:283-300): the halo plane 0 collides a copy of plane nz, nz + 1 of plane 1
+    const long long zc = k == g.nz ? cell_index(g, i, j, 0) : (k == 1 ? cell_index(g, i, j, g.nz + 1) : -1);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double fe = feq_a(q, sr, sux, suy, suz);
+        const double pv = out[q] - (out[q] - fe) / a.tau;
+        __builtin_nontemporal_store(pv, a.post_out + pop_index<M_F64>(q, c, g.ncell));
+        if (zc >= 0) a.post_out[pop_index<M_F64>(q, zc, g.ncell)] = pv;
+    }
+}
+
+hipError_t launch_a_fused(const AFusedArgs &a, hipStream_t s) {
+    const long long total = (long long)a.g.nx * a.g.ny * a.g.nz;
+    k_a_fused<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// Post-collision values of every fluid cell, halo included, of the state f (k_a_collide).
+hipError_t launch_a_collide(const double *f, double *post, const uint8_t *mask, const GridDesc &g, double tau,
+                            hipStream_t s) {
+    const long long all = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+    k_a_collide<<<(unsigned)((all + 255) / 256), 256, 0, s>>>(f, post, mask, g, tau);
+    return hipGetLastError();
+}
+
+// MK_LINE on the listed cells (after clearing it everywhere).
+__global__ __launch_bounds__(256) void k_a_mark_line(uint8_t *mask, long long ncell, const long long *cells,
+                                                     long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ncell) mask[t] &= (uint8_t)~MK_LINE;
+}
+__global__ __launch_bounds__(256) void k_a_set_line(uint8_t *mask, const long long *cells, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) mask[cells[t]] |= MK_LINE;
+}
+
+hipError_t launch_a_mark_line(uint8_t *mask, const GridDesc &g, const long long *cells, long long n, hipStream_t s) {
+    k_a_mark_line<<<(unsigned)((g.ncell + 255) / 256), 256, 0, s>>>(mask, g.ncell, cells, n);
+    if (n > 0) k_a_set_line<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(mask, cells, n);
+    return hipGetLastError();
+}
+
 // 04_hpc_scalability's apply_source (04:212-235) on the listed fluid cells: the value
 // (Amp * spatial) * sin(omega t) (04:229) and the add f += (w_q * value) * dt (04:233-236), in
 // the reference's operation order; sin(omega t) and the spatial factors come from the host

@@ -218,3 +218,34 @@ def test_line_source_is_refused_in_family_b(gpu):
     with pytest.raises(nlos.EngineError):
         e.set_line_source(nlos.reference04_line_source())
     e.close()
+
+
+@pytest.mark.parametrize("chunks", [[1], [2, 1, 17, 4], [24]])
+def test_fused_step_equals_two_pass_step(gpu, chunks, monkeypatch):
+    """nlos_step runs a family-A step as one pass (stream, then the next step's line-source
+    add and collision of the same cell); NLOS_FUSED_A=0 runs 01's two passes.  Both give the
+    same bits (probes, populations, captured field) on 04's scene with its line source,
+    however the steps are split into calls, and across a restart of the source time."""
+    cfg = fa.coarse04_config()
+    solid, sigma = fa.cylinder_04(cfg), fa.radial_sponge_04(cfg)
+    pts = [(0.0, 9.0, 0.0), (7.1, -2.3, 0.4), (-3.3, -11.1, -1.0)]
+    print(f"[family A] fused step, chunks {chunks}", flush=True)
+    runs = []
+    for fused in ("0", "1"):
+        monkeypatch.setenv("NLOS_FUSED_A", fused)
+        src = nlos.reference04_line_source(0)
+        src.dt = 0.577350269 * cfg.dx / 1.0
+        e = _engine(cfg, solid, None, None, pts)
+        e.set_sponge(sigma)
+        e.set_line_source(src)
+        for u, n in enumerate(chunks if fused == "1" else [sum(chunks)]):
+            e.step(n)
+        src.step0 = e.info().step
+        e.set_line_source(src)
+        e.step(5, capture_field=True)
+        e.synchronize()
+        runs.append((e.read_probes(), e.debug_read_f(), e.read_field()))
+        e.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b, equal_nan=True)      # solid cells read as NaN
+    assert np.abs(runs[0][0]).max() > 1e-9               # the line source has reached the probes
