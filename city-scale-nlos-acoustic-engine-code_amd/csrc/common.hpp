@@ -265,6 +265,7 @@ struct AStepArgs {
 // into their periodic z-halo planes 0 and nz + 1), fb_out the next bounce sources.
 constexpr uint8_t MK_LINE = 8;            // family A: fluid cell of the line source (line_factor)
 struct AFusedArgs {
+    const uint32_t *wall_bits;            // per cell: bit b set when c - e_b (ordering B) is solid
     const double *post;
     const double *fb;
     double *f_out;                        // last step (else null)

@@ -933,6 +933,7 @@ int run_a_fused(nlos_engine *e, long long n, bool capture_last) {
         a.post_out = last ? nullptr : post[(s + 1) & 1];
         a.fb_out = last ? nullptr : fbs[(s + 1) & 1];
         a.mask = e->mask.as<uint8_t>();
+        a.wall_bits = e->wall_bits.as<uint32_t>();
         a.sigma = e->sigma_a.as<double>();
         a.line_factor = e->line_factor.as<double>();
         a.amp = e->line.amp;

@@ -27,6 +27,16 @@ constexpr int EYA[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1
 constexpr int EZA[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
 constexpr int INVA[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
 
+// Slot of ordering B (common.hpp) with the velocity of slot q of ordering A.
+constexpr int a2b(int q) {
+    for (int b = 0; b < Q; ++b)
+        if (EX[b] == EXA[q] && EY[b] == EYA[q] && EZ[b] == EZA[q]) return b;
+    return -1;
+}
+constexpr int A2B[Q] = {a2b(0), a2b(1), a2b(2), a2b(3), a2b(4), a2b(5), a2b(6), a2b(7), a2b(8), a2b(9),
+                        a2b(10), a2b(11), a2b(12), a2b(13), a2b(14), a2b(15), a2b(16), a2b(17), a2b(18)};
+static_assert(A2B[0] == 0 && A2B[18] >= 0 && A2B[7] == 7, "ordering A to B");
+
 // get_feq (01:86-91), including the products with zero velocity components.
 __device__ __forceinline__ double feq_a(int k, double r, double ux, double uy, double uz) {
     const double cu = (double)EXA[k] * ux + (double)EYA[k] * uy + (double)EZA[k] * uz;
@@ -170,14 +180,15 @@ __global__ __launch_bounds__(256) void k_a_fused(AFusedArgs a) {
     }
     const double s = a.sigma[n];
     const bool in_sponge = s > 1e-6;
+    // bit b of the wall bits: c - e_b (ordering B) is solid; only cells next to a wall read them
+    const uint32_t wb = (m & MK_NEAR) ? a.wall_bits[c] : 0u;
+    const bool wall = wb != 0u;
     double out[Q];
-    bool wall = false;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const long long src = c - off_a(g, q);
-        if (a.mask[src] & MK_SOLID) {
+        if ((wb >> A2B[q]) & 1u) {
             out[q] = a.fb[pop_index<M_F64>(INVA[q], c, g.ncell)];
-            wall = true;
         } else {
             const double p = a.post[pop_index<M_F64>(q, src, g.ncell)];
             out[q] = in_sponge ? (1.0 - s) * p + s * W64[q] : p;   // feq(q, 1, 0, 0, 0) = w_q
