@@ -112,3 +112,33 @@ if __name__ == "__main__":
     with open(path, "wb") as fh:
         fh.write(reference_city_stl())
     print(path)
+
+
+# ---------------------------------------------------------------------------------------
+# The scene for 08_city_scale_uav.cpp's own run (08:1062-1092): domain x in [0,1650],
+# y in [0,1150], z in [0,250] at DX=1.5, the same STL shift as 05, a 20 Hz source moving
+# from (912, 66.6, 86) along +y at 30 m/s.  Buildings in domain coordinates: three within
+# the first second's wave travel of the source (one next to its path, one pyramid roof),
+# one crossing the y- face close to the source (tracked solid cells that the wave reaches),
+# and blocks crossing the x-, x+ and y+ faces and the top plane.
+REF08_CITY_BOXES = [
+    (850.0, 890.0, 120.0, 170.0, 0.0, 70.0),
+    (940.0, 990.0, 20.0, 60.0, 0.0, 120.0),
+    (1000.0, 1060.0, -20.0, 30.0, 0.0, 40.0),
+    (960.0, 1000.0, 300.0, 340.0, 0.0, 240.0),
+    (-30.0, 40.0, 500.0, 560.0, 0.0, 60.0),
+    (1600.0, 1680.0, 300.0, 360.0, 0.0, 90.0),
+    (400.0, 460.0, 1100.0, 1200.0, 0.0, 300.0),
+]
+REF08_CITY_PYRAMIDS = [
+    (700.0, 760.0, 200.0, 260.0, 0.0, 35.0, 55.0),
+]
+
+
+def reference08_city_stl() -> bytes:
+    tris: List[Tri] = []
+    for b in REF08_CITY_BOXES:
+        tris += box_triangles(*b)
+    for p in REF08_CITY_PYRAMIDS:
+        tris += pyramid_block_triangles(*p)
+    return stl_bytes(shifted(tris, *REF_SHIFT), b"nlos 08 city golden scene")
