@@ -116,7 +116,8 @@ struct StepArgs {
     const long long *probe_cells;        // sorted unique probe cells
     int n_probe_cells;
     double *probe_row;                   // this step's row of the probe history
-    const long long *step_ctr;           // if set: the row is hist + *step_ctr * n_probe_cells
+    const long long *step_ctr;           // if set: the row is hist + (*step_ctr - ctr_lag) * n_probe_cells
+    int ctr_lag;                         // 1 when the counter was advanced before the sweep
     double *hist;
     float *field;                        // interior rho-1 (capture sweeps only)
     int *flag;                           // bit 0: non-finite density seen
@@ -177,6 +178,7 @@ struct GhostArgs {
     long long n_halo;                     // ghost_items(g, skip_zm)
     void *xh;                             // x-halo records or null (see StepArgs)
     int skip_zm;                          // the k = 0 layer is solid ground: not enumerated
+    long long *bump;                      // if set: one thread adds 1 (a device step counter)
 };
 
 // Lanes per halo row of the ghost kernel: the row's cells i in [1, nx], 64-cell aligned.

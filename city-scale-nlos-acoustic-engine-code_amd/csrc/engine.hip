@@ -988,21 +988,24 @@ SourceMulti step_source(const nlos_engine *e, int phase) {
 StepArgs step_args_ctr(const nlos_engine *e, int phase) {
     StepArgs a = step_args(e, phase, e->block);
     a.step_ctr = e->g_ctr.as<long long>();
+    a.ctr_lag = 1;                       // the ghost kernel advanced it (launch_step_ctr)
     a.hist = e->hist.as<double>();
     if (!a.hist) a.n_probe_cells = 0;
     return a;
 }
 
 // One step `phase` steps ahead of the host state: source entries of the step the device
-// counter names, ghost layer, tracked boundary solids, sweep, counter increment.  Leaves
-// the host state as it is (advance_steps moves it).
+// counter names, ghost layer (which also advances the counter), tracked boundary solids,
+// sweep (probe row: the counter minus one).  Leaves the host state as it is
+// (advance_steps moves it).
 int launch_step_ctr(nlos_engine *e, int phase, bool capture) {
     const long long me = stencil_bound(e);
     if (me > 0) {
         const SourceMulti m = step_source(e, phase);
         HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::source_multi(m, me, e->stream); }));
     }
-    const GhostArgs ga = ghost_args(e, phase);
+    GhostArgs ga = ghost_args(e, phase);
+    ga.bump = e->g_ctr.as<long long>();   // the step counter moves on inside the ghost launch
     HIP_TRY(by_mode(e->mode, [&](auto L) { return decltype(L)::ghost(ga, e->lay, e->stream); }));
     // The tracked solids pull from this step's state and ghosts and write only their own
     // next state, which nothing reads before the next step's ghost layer: they run beside
@@ -1020,7 +1023,6 @@ int launch_step_ctr(nlos_engine *e, int phase, bool capture) {
     StepArgs a = step_args_ctr(e, phase);
     a.field = capture ? e->field.as<float>() : nullptr;
     if (int rc = launch_sweep_args(e, a, capture)) return rc;
-    HIP_TRY(launch_count(e->g_ctr.as<long long>(), 1, e->stream));
     if (side) HIP_TRY(hipStreamWaitEvent(e->stream, e->ev_join, 0));
     return NLOS_OK;
 }

@@ -688,7 +688,7 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
         if (!finite) atomicOr(a.flag, 1);
         if (m & MK_PROBE) {
             const int u = probe_slot(a.probe_cells, a.n_probe_cells, c);
-            double *row = a.step_ctr ? a.hist + (*a.step_ctr) * a.n_probe_cells : a.probe_row;
+            double *row = a.step_ctr ? a.hist + (*a.step_ctr - a.ctr_lag) * a.n_probe_cells : a.probe_row;
             if (u >= 0) row[u] = dval;
         }
         if constexpr (CAPTURE) {
@@ -1357,6 +1357,9 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
 
 template <int MODE, int LAY>
 __global__ __launch_bounds__(256) void k_ghost(GhostArgs a) {
+    // the step counter moves on here, after the step's source adds read it and before its
+    // sweep does (the sweep subtracts StepArgs::ctr_lag): one launch fewer per step
+    if (a.bump && blockIdx.x == 0 && threadIdx.x == 0) *a.bump += 1;
     ghost_body<MODE, LAY>(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
