@@ -1209,14 +1209,17 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
     if (ns) {
         mI = a.mask[I];
         // the populations of a fluid neighbour, in every layout
-        if (axis == 0 && a.fb[face & 1]) {
+        // constant indices only: a scene's arguments copied into registers (k_ghost_multi)
+        // would otherwise go to private memory for a runtime index
+        const void *fbf = (face & 1) ? a.fb[1] : a.fb[0];
+        if (axis == 0 && fbf) {
             const long long r0 = ((long long)(jj - 1) + (long long)ny * (kk - 1)) * Q;
             if constexpr (MODE == M_B16) {
-                const float *fb = static_cast<const float *>(a.fb[face & 1]) + r0;
+                const float *fb = static_cast<const float *>(fbf) + r0;
 #pragma unroll
                 for (int q = 0; q < Q; ++q) fI[q] = fb[q];
             } else {
-                const typename Tr<MODE>::S *fb = static_cast<const typename Tr<MODE>::S *>(a.fb[face & 1]) + r0;
+                const typename Tr<MODE>::S *fb = static_cast<const typename Tr<MODE>::S *>(fbf) + r0;
 #pragma unroll
                 for (int q = 0; q < Q; ++q) fI[q] = ld<MODE>(fb, q, a.inv_scale);   // per-thread address
             }
@@ -1239,7 +1242,9 @@ __device__ __forceinline__ void ghost_body(const GhostArgs &a, long long t) {
             if (axis == 0) pidx = (jj - 1) + ny * (kk - 1);
             else if (axis == 1) pidx = (ii - 1) + nx * (kk - 1);
             else pidx = (ii - 1) + nx * (jj - 1);
-            const int slot = a.face_map[face][pidx];
+            const int *fm = face == 0 ? a.face_map[0] : face == 1 ? a.face_map[1] : face == 2 ? a.face_map[2]
+                          : face == 3 ? a.face_map[3] : face == 4 ? a.face_map[4] : a.face_map[5];
+            const int slot = fm[pidx];
             const C *so = static_cast<const C *>(a.s_old);
 #pragma unroll
             for (int q = 0; q < Q; ++q) fI[q] = so[(long long)slot * Q + q];
