@@ -246,7 +246,9 @@ int nlos_exchange_halos(nlos_engine *e);           /* 05:671-904 (single domain:
 int nlos_step_collision(nlos_engine *e);           /* 05:906-972 + probe record 05:1085-1093 */
 /* n fused steps s = step .. step+n-1, each apply_source(s*dt), exchange_halos,
  * step_collision, probe record.  flags: NLOS_STEP_CAPTURE_FIELD keeps rho-1 of the last
- * step for nlos_read_field. */
+ * step for nlos_read_field.  Two-array engines replay a captured HIP graph of two steps
+ * (captured on the first call and again when a buffer it uses grows); family-A engines run
+ * a step as one fused pass.  Both are bit-identical to the separate launches. */
 #define NLOS_STEP_CAPTURE_FIELD 1
 int nlos_step(nlos_engine *e, int64_t n, int32_t flags);
 
