@@ -1101,11 +1101,6 @@ int stage_counter_schedule(nlos_engine *e) {
         HIP_TRY(e->sched_off_d.reserve(std::max<size_t>(nb, 4097 * 8)));
     }
     if (!e->g_ctr.p) HIP_TRY(e->g_ctr.reserve(2 * sizeof(long long)));
-    if (e->n_solid > 0 && !e->side_stream) {
-        HIP_TRY(hipStreamCreateWithFlags(&e->side_stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
-    }
     if (!e->g_src.p) HIP_TRY(e->g_src.reserve(2 * sizeof(SourceArgs)));
     e->h_ctr[0] = e->step;
     e->h_ctr[1] = e->step;
@@ -1279,6 +1274,11 @@ NLOS_API int nlos_create(const nlos_config *cfg, nlos_engine **out) {
     if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(he)));
     he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) return bail(fail(NLOS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)));
+    // the side stream of the tracked boundary solids (launch_step_ctr), on the engine's device
+    if (hipStreamCreateWithFlags(&e->side_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(NLOS_ERR_HIP, "side stream creation failed"));
     const size_t popb = pop_bytes(e);
     e->pattern = e->cfg.stream_pattern;
     if (e->pattern == NLOS_PATTERN_AUTO) {
@@ -2551,6 +2551,12 @@ NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **
         }
         e->batch = b;
         b->scenes.push_back(e);
+        // a scene is stepped by the batch only: its side stream would stay unused
+        (void)hipEventDestroy(e->ev_fork);
+        (void)hipEventDestroy(e->ev_join);
+        (void)hipStreamDestroy(e->side_stream);
+        e->ev_fork = e->ev_join = nullptr;
+        e->side_stream = nullptr;
     }
     *out = b;
     return NLOS_OK;
