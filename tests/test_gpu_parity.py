@@ -199,6 +199,52 @@ def test_inplace_equals_two_array_bitwise(gpu, precision, scene):
         assert np.array_equal(a, b)
 
 
+def _rows_of_three_waves(precision):
+    """Rows of 192 cells, three 64-cell waves: solids ending on both sides of the wave
+    boundaries at x = 64 and 128 and at the row ends, a density field that differs from
+    cell to cell (so a value moved to the wrong lane shows at once) and a moving source."""
+    nx, ny, nz = 192, 20, 16
+    cfg, solid, _, _ = scenes.box_scene(precision, nx=nx, ny=ny, nz=nz)
+    solid[:6, 3:7, 62:66] = 1
+    solid[:9, 9:12, 127:129] = 1
+    solid[:4, 14:18, 188:192] = 1
+    solid[:3, 0:3, 0:2] = 1
+    solid[5:8, 12:15, 63:64] = 1
+    rng = np.random.default_rng(11)
+    rho = 1.0 + 1e-3 * rng.random((nz, ny, nx))
+    src = nlos.default_source(sx=60.3, sy=9.7, sz=6.2, ex=131.1, ey=12.4, ez=8.0, speed=900.0,
+                              amp=2e-2, freq=120.0, waveform=nlos.WAVE_SINE)
+    probes = [(63.5, 10.5, 4.5), (64.5, 5.5, 9.5), (127.5, 12.5, 2.5), (190.5, 3.5, 12.5),
+              (0.5, 18.5, 6.5)]
+    return cfg, solid, src, probes, rho
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "f16s"])
+def test_inplace_rows_of_whole_waves(gpu, precision):
+    """Rows a whole number of 64-cell waves, solids at the wave boundaries and a density
+    that differs from cell to cell: the in-place A-A pattern equals the two-array pattern
+    bit for bit (partial reflection in FP64 and FP32; scaled FP16 with partial reflection
+    rounds the wall copies in another order in the two patterns, so it runs at 1), and
+    FP64 equals the oracle."""
+    cfg, solid, src, probes, rho = _rows_of_three_waves(precision)
+    cfg.wall_reflect = 1.0 if precision == "f16s" else 0.8
+    runs = []
+    for pattern in ("inplace", "two_array"):
+        print(f"[rows of whole waves] {precision} {pattern}", flush=True)
+        cfg.stream_pattern = nlos.PATTERNS[pattern]
+        e = _engine_run(cfg, solid, src, probes, 41, rho)
+        runs.append((e.read_probes(), e.read_field(), e.debug_read_f()[solid == 0]))
+        e.close()
+    assert np.abs(runs[0][1]).max() > 0
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+    if precision == "f64":
+        o = _oracle_run(cfg, solid, src, probes, 41, rho)
+        assert np.array_equal(runs[0][0], o.read_probes())
+        assert np.array_equal(runs[0][1], o.read_field())
+        assert np.array_equal(runs[0][2], _oracle_interior_f(o)[solid == 0])
+
+
 def test_inplace_and_two_array_fp16_reflection_agree_to_rounding(gpu):
     """Scaled FP16 with wall_reflect = 0.8: the two patterns round the reflected wall values
     in a different order, so they agree to FP16 rounding noise, far inside the FP16S
