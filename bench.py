@@ -269,8 +269,10 @@ def bench_batch(args) -> None:
         e.set_probes(d.probes)
     log(f"batch: {len(defs)} scenes, {cells} cells, set up in {time.perf_counter() - t:.1f} s")
     tune = b.autotune(reps=3)
-    log(f"autotune: {tune.block_x}x{tune.block_y}, {tune.best_ms:.3f} ms "
-        f"({tune.candidates} shapes)")
+    widths = b.tile_widths()
+    width_count = {str(w): widths.count(w) for w in sorted(set(widths))}
+    log(f"autotune: {'per row-length class' if tune.per_scene else f'{tune.block_x}x{tune.block_y}'}, "
+        f"scenes per tile width {width_count}, {tune.best_ms:.3f} ms ({tune.candidates} sweeps timed)")
     b.step(2)                      # graph capture outside the timed region
     b.synchronize()
     b.timing(True)
@@ -320,7 +322,8 @@ def bench_batch(args) -> None:
            "config": {"workload": f"{len(defs)} scenes, dims U[64,192]x[64,192]x[64,96], {cells} cells, "
                                   "point sources 2-20 m over reflecting ground (0.6-1.0), half moving "
                                   "1-10 m/s, 4 probes each"},
-           "autotune": {"block": [tune.block_x, tune.block_y],
+           "autotune": {"block": [tune.block_x, tune.block_y], "per_scene": bool(tune.per_scene),
+                        "scenes_per_tile_width": width_count,
                         "candidates": tune.candidates, "sweep_ms": round(tune.best_ms, 4)},
            "graph_ms_per_step": round(tm.avg_ms, 4),
            "one_scene_at_a_time_mlups": round(seq_mlups, 1),

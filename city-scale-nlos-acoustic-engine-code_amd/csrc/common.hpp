@@ -124,6 +124,8 @@ struct StepArgs {
     int tiles_x, tiles_y;                // block tiling of the interior
     int nblocks;
     int dpp;                             // every lane active: odd sweep may shift across lanes
+    int tile_w;                          // multi-scene sweep: tile width 64, 128 or 256 (tiles
+                                         // of tile_w x 256 / tile_w cells; tiles_x, tiles_y match)
     void *fb[2];                         // x-face buffers (x-, x+) or null, see below
     const uint32_t *wall_bits;           // two-array sweep: bit q set when c - e_q is solid
     const void *xh;                      // x-halo records or null, see below

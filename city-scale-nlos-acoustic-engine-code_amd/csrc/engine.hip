@@ -2316,7 +2316,8 @@ struct nlos_batch {
     int device = 0;
     int n_cu = 0;
     hipStream_t stream = nullptr;
-    dim3 block{256, 1, 1};
+    dim3 block{256, 1, 1};                // sweep tile shape of every scene without its own width
+    std::vector<int> tile_w;              // per-scene sweep tile widths (64, 128, 256), or empty
     DevBuf step_args[2], ghost_args[2], solid_args[2], source_args[2];
     DevBuf tile_table, halo_blocks, solid_blocks, ctr;
     long long total_tiles = 0;            // sweep grid x: the largest scene's tile count
@@ -2344,6 +2345,16 @@ int batch_ready(const nlos_batch *b) {
 int batch_settle(const nlos_batch *b) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     return NLOS_OK;
+}
+
+// Sweep tile width of scene i: tiles of w x 256 / w cells, w = 64, 128 or 256.
+int scene_tile_w(const nlos_batch *b, size_t i) { return b->tile_w.empty() ? (int)b->block.x : b->tile_w[i]; }
+
+// Sweep arguments of scene i in the given phase for tile width w.
+StepArgs multi_step_args(const nlos_batch *b, size_t i, int phase, int w) {
+    StepArgs a = step_args(b->scenes[i], phase, dim3(w, 256 / w, 1));
+    a.tile_w = w;
+    return a;
 }
 
 
@@ -2385,12 +2396,13 @@ int batch_prepare(nlos_batch *b, bool &moved) {
     std::vector<int> tiles;                // sweep tiles per scene
     std::vector<int2> halo, solid;         // work tables (scene, first item)
     b->max_entries = 0;
-    for (nlos_engine *e : b->scenes) {
+    for (size_t si = 0; si < b->scenes.size(); ++si) {
+        nlos_engine *e = b->scenes[si];
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (int rc = sync_face_buffers(e)) return rc;
         HIP_TRY(hipStreamSynchronize(e->stream));
         for (int ph = 0; ph < 2; ++ph) {
-            StepArgs a = step_args(e, ph, b->block);
+            StepArgs a = multi_step_args(b, si, ph, scene_tile_w(b, si));
             a.step_ctr = b->ctr.as<long long>();
             a.hist = e->hist.as<double>();
             if (!a.hist) a.n_probe_cells = 0;
@@ -2449,7 +2461,7 @@ int batch_launch_step(nlos_batch *b, int phase) {
                                 (int)b->total_solid, b->stream);
         if (r == hipSuccess)
             r = LL::step_ab_multi(b->step_args[phase].as<StepArgs>(), b->tile_table.as<int>(), n,
-                                  (int)b->total_tiles, b->block, b->stream);
+                                  (int)b->total_tiles, b->stream);
         return r;
     }));
     HIP_TRY(launch_count(b->ctr.as<long long>(), 1, b->stream));
@@ -2599,57 +2611,122 @@ NLOS_API int nlos_batch_scene(nlos_batch *b, int32_t i, nlos_engine **scene) {
 NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result) {
     if (int rc = batch_ready(b)) return rc;
     if (reps < 1) return fail(NLOS_ERR_ARG, "reps must be positive");
-    const dim3 blocks[] = {dim3(64, 4, 1), dim3(128, 2, 1), dim3(256, 1, 1)};
+    const int widths[] = {64, 128, 256};
+    const size_t n = b->scenes.size();
     if (!b->ev0) HIP_TRY(hipEventCreate(&b->ev0));
     if (!b->ev1) HIP_TRY(hipEventCreate(&b->ev1));
-    double best = 1e300;
-    dim3 best_block = b->block;
-    int tried = 0;
     bool moved = false;
-    for (const dim3 &bl : blocks) {
-        b->block = bl;
-        if (int rc = batch_prepare(b, moved)) return rc;
-        // the sweep alone, into the array the next step overwrites: no probe rows, no
-        // face records
+    if (int rc = batch_prepare(b, moved)) return rc;   // face records in sync
+    int tried = 0;
+    // Sweep time of the scenes `sel` with tile widths `w` (the fastest of reps launches):
+    // the sweep alone, into the
+    // array the next step overwrites (no probe rows, no face records), after one untimed
+    // launch.  The argument and tile arrays are rewritten by the next batch_prepare.
+    auto time_sweep = [&](const std::vector<int> &sel, const std::vector<int> &w, double &ms_out) -> int {
         std::vector<StepArgs> st;
-        for (nlos_engine *e : b->scenes) {
-            StepArgs a = step_args(e, 0, b->block);
+        std::vector<int> tiles;
+        for (size_t k = 0; k < sel.size(); ++k) {
+            StepArgs a = multi_step_args(b, (size_t)sel[k], 0, w[k]);
             a.n_probe_cells = 0;
             a.fb[0] = a.fb[1] = nullptr;
             st.push_back(a);
+            tiles.push_back(a.nblocks);
         }
-        if (int rc = upload_vec(b, b->step_args[0], st, moved)) return rc;
+        bool mv = false;
+        if (int rc = upload_vec(b, b->step_args[0], st, mv)) return rc;
+        if (int rc = upload_vec(b, b->tile_table, tiles, mv)) return rc;
         HIP_TRY(hipStreamSynchronize(b->stream));
+        const int max_tiles = tiles.empty() ? 0 : *std::max_element(tiles.begin(), tiles.end());
         auto sweep = [&]() {
             return by_mode(b->mode, [&](auto L) {
                 return decltype(L)::step_ab_multi(b->step_args[0].as<StepArgs>(), b->tile_table.as<int>(),
-                                                  (int)b->scenes.size(), (int)b->total_tiles, b->block,
-                                                  b->stream);
+                                                  (int)sel.size(), max_tiles, b->stream);
             });
         };
         HIP_TRY(sweep());
-        HIP_TRY(hipEventRecord(b->ev0, b->stream));
-        for (int r = 0; r < reps; ++r) HIP_TRY(sweep());
-        HIP_TRY(hipEventRecord(b->ev1, b->stream));
-        HIP_TRY(hipEventSynchronize(b->ev1));
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        ms_out = 1e300;
+        for (int r = 0; r < reps; ++r) {   // the fastest launch: the least disturbed one
+            HIP_TRY(hipEventRecord(b->ev0, b->stream));
+            HIP_TRY(sweep());
+            HIP_TRY(hipEventRecord(b->ev1, b->stream));
+            HIP_TRY(hipEventSynchronize(b->ev1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            ms_out = std::min(ms_out, (double)ms);
+        }
         ++tried;
-        if (ms / reps < best) {
-            best = ms / reps;
-            best_block = bl;
+        return NLOS_OK;
+    };
+    std::vector<int> all(n);
+    for (size_t i = 0; i < n; ++i) all[i] = (int)i;
+    // one width for every scene
+    double best = 1e300;
+    int best_w = (int)b->block.x;
+    for (int w : widths) {
+        double ms = 0.0;
+        if (int rc = time_sweep(all, std::vector<int>(n, w), ms)) return rc;
+        if (ms < best) { best = ms; best_w = w; }
+    }
+    // per class of row length (interior rows of 1, 2, 3 or more 64-cell waves): each class
+    // timed alone with every width, then all scenes with the classes' choices
+    std::vector<std::vector<int>> cls(4);
+    for (size_t i = 0; i < n; ++i)
+        cls[std::min(3, (b->scenes[i]->g.nx + 63) / 64 - 1)].push_back((int)i);
+    int n_cls = 0;
+    for (const auto &c : cls) n_cls += c.empty() ? 0 : 1;
+    std::vector<int> per(n, best_w);
+    bool use_per = false;
+    double per_ms = 1e300;
+    if (n_cls > 1) {
+        for (const auto &c : cls) {
+            if (c.empty()) continue;
+            double cb = 1e300;
+            int cw = best_w;
+            for (int w : widths) {
+                double ms = 0.0;
+                if (int rc = time_sweep(c, std::vector<int>(c.size(), w), ms)) return rc;
+                if (ms < cb) { cb = ms; cw = w; }
+            }
+            for (int i : c) per[(size_t)i] = cw;
+        }
+        if (per != std::vector<int>(n, best_w)) {
+            if (int rc = time_sweep(all, per, per_ms)) return rc;
+            use_per = per_ms < 0.99 * best;   // a gain above launch-to-launch noise
         }
     }
-    b->block = best_block;
+    if (use_per) {
+        b->tile_w = per;
+        best = per_ms;
+    } else {
+        b->tile_w.clear();
+        b->block = dim3(best_w, 256 / best_w, 1);
+    }
     drop_graph(b);
     b->baked.clear();
     if (result) {
-        result->block_x = (int32_t)best_block.x;
-        result->block_y = (int32_t)best_block.y;
+        result->block_x = use_per ? 0 : (int32_t)best_w;
+        result->block_y = use_per ? 0 : (int32_t)(256 / best_w);
         result->candidates = tried;
-        result->reserved = 0;
+        result->per_scene = use_per ? 1 : 0;
         result->best_ms = best;
     }
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_tile_widths(const nlos_batch *b, int32_t *widths, int32_t n) {
+    if (int rc = batch_ready(b)) return rc;
+    if (!widths || n != (int32_t)b->scenes.size()) return fail(NLOS_ERR_ARG, "need one width per scene");
+    for (size_t i = 0; i < b->scenes.size(); ++i) widths[i] = scene_tile_w(b, i);
+    return NLOS_OK;
+}
+
+NLOS_API int nlos_batch_set_tile_widths(nlos_batch *b, const int32_t *widths, int32_t n) {
+    if (int rc = batch_ready(b)) return rc;
+    if (!widths || n != (int32_t)b->scenes.size()) return fail(NLOS_ERR_ARG, "need one width per scene");
+    for (int32_t i = 0; i < n; ++i)
+        if (widths[i] != 64 && widths[i] != 128 && widths[i] != 256)
+            return fail(NLOS_ERR_ARG, "tile width %d of scene %d is not 64, 128 or 256", widths[i], i);
+    b->tile_w.assign(widths, widths + n);
     return NLOS_OK;
 }
 
@@ -2675,9 +2752,8 @@ NLOS_API int nlos_batch_step(nlos_batch *b, int64_t n) {
         b->h_ctr[1] = b->step;
         HIP_TRY(hipMemcpyAsync(b->ctr.p, b->h_ctr, sizeof b->h_ctr, hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
-        const std::vector<long long> shape = {
-            (long long)b->block.x, (long long)b->block.y, b->total_tiles,
-            b->total_halo, b->total_solid, b->max_entries};
+        // the tile shapes are read from the argument arrays; only the grids are baked
+        const std::vector<long long> shape = {b->total_tiles, b->total_halo, b->total_solid, b->max_entries};
         if (moved || shape != b->baked) drop_graph(b);
         if (m >= 2 && !b->exec && b->use_graph) {
             if (int rc = batch_capture(b)) return rc;

@@ -746,25 +746,27 @@ __device__ __forceinline__ void ab_cell(const StepArgs &a, const AbIn<MODE> &in,
     }
 }
 
-// Tile b: (bi, bj) of the x-y plane at plane k.
+// Tile b: (bi, bj) of the x-y plane at plane k, tw x th cells; the thread sweeps cell
+// (tx, ty) of it.  tw is a multiple of 64 and the threads of a wave are consecutive in tx,
+// so a wave is one 64-cell row segment.
 template <int MODE, bool CAPTURE, bool VEL = false>
-__device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
+__device__ __forceinline__ void step_ab_body(const StepArgs &a, int b, int tx, int ty, int tw, int th) {
     using C = typename Tr<MODE>::C;
     using S = typename Tr<MODE>::S;
     const int bi = b % a.tiles_x;
     const int rest_b = b / a.tiles_x;
     const int bj = rest_b % a.tiles_y;
     const int k = rest_b / a.tiles_y + 1;
-    const int i0 = bi * (int)blockDim.x + 1;
-    const int j0 = bj * (int)blockDim.y + 1;
-    const int i = i0 + (int)threadIdx.x;
-    const int j = j0 + (int)threadIdx.y;
+    const int i0 = bi * tw + 1;
+    const int j0 = bj * th + 1;
+    const int i = i0 + tx;
+    const int j = j0 + ty;
     if (i > a.g.nx || j > a.g.ny) return;
     const long long N = a.g.ncell;
     // cell = tile corner (the same for the whole workgroup) + lane offset (32-bit): the
     // compiler then forms every slot address as a scalar base plus a 32-bit lane offset
     const long long cb = cell_index(a.g, i0, j0, k);
-    const unsigned t = threadIdx.x + (unsigned)a.g.px * threadIdx.y;
+    const unsigned t = (unsigned)tx + (unsigned)a.g.px * (unsigned)ty;
     const long long c = cb + t;
     constexpr int S0 = MODE == M_B16 ? 1 : 0;   // see ab_cell
     const std::conditional_t<MODE == M_B16, Lanes<M_B16, Q - 1>, Lanes<MODE>> L(cb, t, N);
@@ -816,7 +818,8 @@ __device__ __forceinline__ void step_ab_body(const StepArgs &a, int b) {
 
 template <int MODE, bool CAPTURE, bool VEL = false>
 __global__ __launch_bounds__(256) void k_step_ab(StepArgs a) {
-    step_ab_body<MODE, CAPTURE, VEL>(a, blockIdx.x);
+    step_ab_body<MODE, CAPTURE, VEL>(a, blockIdx.x, (int)threadIdx.x, (int)threadIdx.y, (int)blockDim.x,
+                                     (int)blockDim.y);
 }
 
 // Sweep over the tiles of several scenes: workgroup (t, s) sweeps tile t of scene s
@@ -829,7 +832,11 @@ __global__ __launch_bounds__(256) void k_step_ab_multi(const StepArgs *__restric
     // a private copy: read through the pointer, every field would be reloaded after each
     // store of the sweep (the compiler cannot rule out aliasing)
     const StepArgs a = scenes[sc];
-    step_ab_body<MODE, false>(a, blockIdx.x);
+    // one-dimensional workgroups of 256 threads; each scene's own tile shape, tile_w x
+    // (256 / tile_w) cells with tile_w = 64, 128 or 256
+    const int lw = 31 - __builtin_clz((unsigned)a.tile_w);
+    step_ab_body<MODE, false>(a, blockIdx.x, (int)(threadIdx.x & ((unsigned)a.tile_w - 1u)), (int)(threadIdx.x >> lw),
+                              a.tile_w, 256 >> lw);
 }
 
 // Wall copies of LAY_P (see Layout): for every fluid cell c next to a wall and every q
@@ -1760,9 +1767,9 @@ hipError_t Launch<MODE>::fb_gather(const void *pop, const uint8_t *mask, const G
 
 template <int MODE>
 hipError_t Launch<MODE>::step_ab_multi(const StepArgs *scenes, const int *n_tiles, int n_scenes, int max_tiles,
-                                       dim3 block, hipStream_t s) {
+                                       hipStream_t s) {
     if (n_scenes <= 0 || max_tiles <= 0) return hipSuccess;
-    k_step_ab_multi<MODE><<<dim3(max_tiles, n_scenes), block, 0, s>>>(scenes, n_tiles);
+    k_step_ab_multi<MODE><<<dim3(max_tiles, n_scenes), 256, 0, s>>>(scenes, n_tiles);
     return hipGetLastError();
 }
 

@@ -42,7 +42,7 @@ struct Launch {
     // per scene); blocks[g] = (scene, first item) for the 256-thread workgroups of the
     // ghost and boundary-solid launches.
     static hipError_t step_ab_multi(const StepArgs *scenes, const int *n_tiles, int n_scenes, int max_tiles,
-                                    dim3 block, hipStream_t s);
+                                    hipStream_t s);
     static hipError_t ghost_multi(const GhostArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
     static hipError_t solid_multi(const SolidArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
     // Source entries of the step named by the device counter; at most max_entries per step.

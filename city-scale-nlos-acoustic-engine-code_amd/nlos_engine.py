@@ -94,7 +94,7 @@ def reference04_line_source(step0: int = 0) -> LineSource:
 
 class Tune(C.Structure):
     _fields_ = [("block_x", C.c_int32), ("block_y", C.c_int32), ("candidates", C.c_int32),
-                ("reserved", C.c_int32), ("best_ms", C.c_double)]
+                ("per_scene", C.c_int32), ("best_ms", C.c_double)]
 
 
 # Every C-ABI entry point of include/nlos_engine.h (plus the test hooks of
@@ -144,6 +144,8 @@ SIGNATURES = {
     "nlos_batch_size": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "nlos_batch_scene": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_void_p)]),
     "nlos_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(Tune)]),
+    "nlos_batch_tile_widths": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
+    "nlos_batch_set_tile_widths": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
     "nlos_batch_step": (C.c_int, [_P, C.c_int64]),
     "nlos_batch_synchronize": (C.c_int, [_P]),
     "nlos_batch_timing_enable": (C.c_int, [_P, C.c_int32]),
@@ -545,6 +547,16 @@ class Batch:
         t = Tune()
         _check(self.lib.nlos_batch_autotune(self.h, reps, C.byref(t)))
         return t
+
+    def tile_widths(self) -> List[int]:
+        """Sweep tile width of every scene (tiles of w x 256/w cells)."""
+        w = (C.c_int32 * len(self.scenes))()
+        _check(self.lib.nlos_batch_tile_widths(self.h, w, len(self.scenes)))
+        return list(w)
+
+    def set_tile_widths(self, widths: Sequence[int]) -> None:
+        w = (C.c_int32 * len(widths))(*widths)
+        _check(self.lib.nlos_batch_set_tile_widths(self.h, w, len(widths)))
 
     def step(self, n: int) -> None:
         _check(self.lib.nlos_batch_step(self.h, int(n)))

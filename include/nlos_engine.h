@@ -361,10 +361,12 @@ int nlos_backward(nlos_engine *e, const double *dl_dprobe, int64_t max_steps, nl
 typedef struct nlos_batch nlos_batch;
 
 typedef struct nlos_tune {
-    int32_t block_x, block_y;   /* workgroup shape of the sweep (block_x * block_y = 256) */
-    int32_t candidates;         /* shapes timed */
-    int32_t reserved;
-    double best_ms;             /* sweep time of the chosen shape, all scenes, one step */
+    int32_t block_x, block_y;   /* one tile shape for every scene (block_x * block_y = 256),
+                                   or 0, 0 when per_scene is set */
+    int32_t candidates;         /* sweeps timed */
+    int32_t per_scene;          /* 1: each class of row length has its own tile width
+                                   (nlos_batch_tile_widths) */
+    double best_ms;             /* sweep time of the choice, all scenes, one step */
 } nlos_tune;
 
 int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **out);
@@ -372,10 +374,17 @@ int nlos_batch_destroy(nlos_batch *b);
 int nlos_batch_size(const nlos_batch *b, int32_t *n);
 /* The engine handle of scene i (owned by the batch). */
 int nlos_batch_scene(nlos_batch *b, int32_t i, nlos_engine **scene);
-/* Times the sweep of all scenes for each candidate workgroup shape (reps
- * launches each, into the array the next step overwrites, so the state is untouched) and
- * keeps the fastest.  result may be NULL. */
+/* Picks the sweep tile shapes: tiles of w x 256/w cells, w = 64, 128 or 256.  Times the
+ * sweep of all scenes with each w, then each class of row length (interior rows of 1, 2, 3,
+ * or more 64-cell waves) alone with each w, then all scenes with every class at its best w,
+ * which is kept when it is at least 1 % faster than the best single w (each timing the
+ * fastest of reps launches, into the array the next step overwrites, so the state is
+ * untouched).  result may be NULL. */
 int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result);
+/* The sweep tile width of every scene (n = the batch size), and setting them (each 64, 128
+ * or 256), e.g. to reuse an autotuned choice. */
+int nlos_batch_tile_widths(const nlos_batch *b, int32_t *widths, int32_t n);
+int nlos_batch_set_tile_widths(nlos_batch *b, const int32_t *widths, int32_t n);
 /* n steps of every scene (each: apply_source(s*dt), exchange_halos, step_collision, probe
  * record, as nlos_step). */
 int nlos_batch_step(nlos_batch *b, int64_t n);

@@ -79,11 +79,48 @@ def test_batch_autotune_keeps_state_and_results(gpu):
         _setup(b.scene(i), d)
     b.step(3)
     t = b.autotune(reps=1)
-    assert t.candidates == 3 and t.block_x * t.block_y == 256 and t.best_ms > 0.0
+    # three widths for all scenes, then three for each of the two row-length classes
+    # (rows of one wave: nx = 40, 30, 12, 28; of two: nx = 100), and the classes' choice
+    # timed together when it differs from the best single width
+    assert 9 <= t.candidates <= 10 and t.best_ms > 0.0
+    w = b.tile_widths()
+    assert len(w) == len(defs) and set(w) <= {64, 128, 256}
+    if t.per_scene:
+        assert t.block_x == 0 and t.block_y == 0 and len(set(w)) > 1
+    else:
+        assert t.block_x * t.block_y == 256 and w == [t.block_x] * len(defs)
     b.step(6)
     b.synchronize()
     for i, d in enumerate(defs):
         want_p, want_f = _single(d, 9)
+        assert np.array_equal(b.scene(i).read_probes(), want_p), i
+        assert np.array_equal(b.scene(i).debug_read_f(), want_f, equal_nan=True), i
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f16s", "b16"])
+def test_batch_with_a_tile_width_per_scene(gpu, precision):
+    """Scenes of one launch sweeping with different tile shapes (64x4, 128x2, 256x1 cells)
+    still give the bits of each scene stepped alone; widths other than 64, 128, 256 are
+    refused."""
+    print(f"[batch, per-scene tile widths] precision={precision}", flush=True)
+    defs = _defs(precision)
+    b = nlos.Batch([d[0] for d in defs])
+    for i, d in enumerate(defs):
+        _setup(b.scene(i), d)
+    widths = [64, 128, 256, 128, 64]
+    b.set_tile_widths(widths)
+    assert b.tile_widths() == widths
+    with pytest.raises(nlos.EngineError):
+        b.set_tile_widths([64, 96, 256, 128, 64])
+    with pytest.raises(nlos.EngineError):
+        b.set_tile_widths([64, 128])
+    b.step(7)
+    b.set_tile_widths([256, 64, 128, 64, 256])   # a new choice between two step calls
+    b.step(10)
+    b.synchronize()
+    for i, d in enumerate(defs):
+        want_p, want_f = _single(d, 17)
         assert np.array_equal(b.scene(i).read_probes(), want_p), i
         assert np.array_equal(b.scene(i).debug_read_f(), want_f, equal_nan=True), i
     b.close()
