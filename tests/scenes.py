@@ -105,3 +105,50 @@ def canyon_scene(precision="f64", nx=28, ny=24, nz=14, waveform="ricker", n_samp
     if open_faces:
         probes += [(0.5, c + 0.5, 3.5), (13.5, ny - 0.5, 11.5)]
     return cfg, solid, src, src_samples, probes
+
+
+def random_scene(seed, precision="f64"):
+    """A seeded scene of random shape and options: grid 1-200 x 2-40 x 3-28 cells at a random
+    origin and dx, random tau, sponge width/strength/faces, NSCBC faces, ground, sky height,
+    wall reflection, solid boxes (some crossing faces), a moving source with a random
+    waveform and injection, probes (some inside solids), and sometimes an initial density
+    field.  Returns cfg, solid, src, samples, probes, rho, steps."""
+    rng = np.random.default_rng(1000 + seed)
+    nx = int(rng.choice([int(rng.integers(1, 5)), int(rng.integers(5, 73)), int(rng.integers(100, 201))]))
+    ny = int(rng.integers(2, 41))
+    nz = int(rng.integers(3, 29))
+    dx = float(rng.choice([0.5, 1.0, 1.5]))
+    x0, y0, z0 = (float(v) for v in rng.uniform(-20.0, 20.0, 3))
+    faces = lambda: int(rng.integers(0, 64))
+    cfg = nlos.default_config(dx=dx, x_min=x0, x_max=x0 + nx * dx, y_min=y0, y_max=y0 + ny * dx,
+                              z_min=z0, z_max=z0 + nz * dx,
+                              sponge_width=float(rng.uniform(0.0, 6.0)) * dx,
+                              sponge_strength=float(rng.uniform(0.05, 0.4)),
+                              tau=float(rng.uniform(0.5005, 0.9)),
+                              sky_z=float(rng.choice([1e9, z0 + rng.uniform(0.3, 0.8) * nz * dx])),
+                              wall_reflect=float(rng.choice([1.0, rng.uniform(0.5, 1.0)])),
+                              sponge_faces=faces(), nscbc_faces=faces(), ground=int(rng.integers(0, 2)),
+                              precision=precision)
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    for _ in range(int(rng.integers(0, 7))):
+        i0, j0 = int(rng.integers(-2, nx)), int(rng.integers(-2, ny))
+        w, d, h = (int(v) for v in rng.integers(1, 9, 3))
+        k0 = int(rng.choice([0, rng.integers(0, nz)]))
+        solid[k0:k0 + h, max(j0, 0):j0 + d, max(i0, 0):i0 + w] = 1
+    inside = lambda: (x0 + rng.uniform(0.05, 0.95) * nx * dx, y0 + rng.uniform(0.05, 0.95) * ny * dx,
+                      z0 + rng.uniform(0.05, 0.95) * nz * dx)
+    s, e = inside(), inside()
+    waveform = int(rng.choice([nlos.WAVE_GAUSS_PULSE, nlos.WAVE_SINE, nlos.WAVE_SAMPLED, nlos.WAVE_RICKER]))
+    dt = dx / (np.sqrt(3.0) * 343.0)
+    steps = int(rng.integers(25, 51))
+    src = nlos.default_source(sx=s[0], sy=s[1], sz=s[2], ex=e[0], ey=e[1], ez=e[2],
+                              speed=float(rng.uniform(0.0, 400.0)), amp=float(rng.uniform(5e-3, 5e-2)),
+                              freq=float(rng.uniform(50.0, 400.0)), t0=float(rng.uniform(0.0, 20.0)) * dt,
+                              sigma_t=float(rng.uniform(3.0, 10.0)) * dt, waveform=waveform,
+                              injection=int(rng.integers(0, 2)))
+    samples = rng.standard_normal(steps + 8) if waveform == nlos.WAVE_SAMPLED else None
+    probes = [inside() for _ in range(int(rng.integers(1, 7)))]
+    rho = None
+    if rng.random() < 0.3:
+        rho = 1.0 + 1e-3 * rng.random((nz, ny, nx))
+    return cfg, solid, src, samples, probes, rho, steps
