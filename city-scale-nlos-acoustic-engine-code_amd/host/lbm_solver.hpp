@@ -11,7 +11,7 @@
 // Fields beyond the reference's constants (all default to the reference's values):
 // c_phys, sponge width/strength/faces, sky_z, NSCBC faces, ground, wall reflection,
 // precision, waveform (pulse of 05 or sine of 08_city_scale_uav.cpp:543), injection,
-// write_frames.
+// write_frames, binary_frames, max_frames.
 #pragma once
 
 #include <algorithm>
@@ -74,6 +74,7 @@ public:
     bool write_frames = true;
     bool async_frames = true;         // frames written off the step stream (same files)
     bool binary_frames = false;       // legacy VTK BINARY instead of the reference's ASCII
+    int max_frames = -1;              // >= 0: run stops once this many frames are written
 
     LBMSolver() = default;
     LBMSolver(const LBMSolver &) = delete;
@@ -163,6 +164,7 @@ public:
             check(nlos_step(eng_, next - s + 1, frame ? NLOS_STEP_CAPTURE_FIELD : 0));
             s = next + 1;
             if (frame) write_vtk(frame_id++);
+            if (max_frames >= 0 && frame_id >= max_frames) break;
         }
         check(nlos_dump_wait(eng_));
         collect_probes();
