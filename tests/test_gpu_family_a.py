@@ -71,6 +71,55 @@ def test_ref01_vtk_frames(gpu, tmp_path):
         assert got == want["values"], f
 
 
+def test_cpp_acoustic_scattering_program_reproduces_reference(gpu, tmp_path):
+    """host/acoustic_scattering.cpp: 01's run through the C ABI.  Its probe files equal the
+    compiled 01's byte for byte, its frames at steps 104 and 208 hold 01's sampled values, and
+    its frame of step 0 is the initial pulse in 01's format."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(nlos.LIB_PATH), "acoustic_scattering")
+    assert os.path.exists(exe), "build the package (make) first"
+    out = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    for name in ("A", "B", "C"):
+        got = (tmp_path / "output_mpi" / f"probe_{name}.dat").read_text()
+        assert got == open(os.path.join(GOLDEN, f"probe_{name}.dat")).read(), name
+    frames = json.load(open(os.path.join(GOLDEN, "vtk_frames.json")))
+    nx, ny, nz = frames["grid"]
+    for f in ("104", "208"):
+        want = frames["frames"][f]
+        lines = (tmp_path / "output_mpi" / f"cyl_{int(f):05d}.vtk").read_text().split("\n")
+        assert lines[:10] == want["header"], f
+        k, st = want["layer"], want["stride"]
+        got = [lines[10 + (k * ny + j) * nx + i] for j in range(0, ny, st) for i in range(0, nx, st)]
+        assert got == want["values"], f
+    lines = (tmp_path / "output_mpi" / "cyl_00000.vtk").read_text().split("\n")
+    assert lines[:10] == frames["frames"]["104"]["header"]
+    cfg = fa.config()
+    solid, rho = fa.cylinder_and_pulse(cfg)
+    rho[solid == 1] = 1.0
+    assert lines[10:-1] == ["%.6e" % ((v - 1.0) * 100.0) for v in rho.ravel()]
+
+
+def test_cpp_hpc_scalability_program_prints_04s_report(gpu, tmp_path):
+    """host/hpc_scalability.cpp: 04's benchmark (208^3, line source, 100 + 2000 steps) through
+    the C ABI, reporting as 04 does (04:366-401).  04 prints only its timing, so this checks
+    the report and a rate no CPU run of 04 reaches."""
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(nlos.LIB_PATH), "hpc_scalability")
+    assert os.path.exists(exe), "build the package (make) first"
+    out = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    assert lines[0] == "=== START BENCHMARK ===" and lines[-1] == "=== END BENCHMARK ==="
+    assert "Grid: 208 x 208 x 208 (dx=0.125)" in lines
+    mlups = float(re.search(r"^MLUPS: (\S+)$", out.stdout, re.M).group(1))
+    secs = float(re.search(r"^BENCHMARK_TIME: (\S+) s$", out.stdout, re.M).group(1))
+    print(f"[04 C++ program] {mlups:.0f} MLUPS, {secs:.3f} s", flush=True)
+    assert abs(208 ** 3 * 2000 / secs / 1e6 - mlups) <= 1e-3 * mlups
+    assert mlups > 2000.0
+
+
 def _small_scene(seed=4):
     cfg = fa.config(x=(-1.2, 1.2), y=(-1.0, 1.0), z=(-0.15, 0.15), dx=0.05, tau=0.52, sponge=0.3, strength=0.8)
     nx, ny, nz = fa.dims(cfg)
