@@ -84,3 +84,49 @@ def test_random_scenes_batched_equal_alone(gpu, precision):
         assert np.array_equal(b.scene(i).read_probes(), want[0]), i
         assert np.array_equal(b.scene(i).debug_read_f()[solid == 0], want[2]), i
     b.close()
+
+
+@pytest.mark.parametrize("seed", SEEDS[:20])
+def test_random_scene_fp16_inplace_equals_two_array(gpu, seed):
+    """Scaled FP16 without partial reflection (with it the two patterns round the wall copies
+    in another order): in place equals two arrays bit for bit."""
+    cfg, solid, src, samples, probes, rho, steps = scenes.random_scene(seed, "f16s")
+    cfg.wall_reflect = 1.0
+    runs = []
+    for pattern in ("two_array", "inplace"):
+        cfg.stream_pattern = nlos.PATTERNS[pattern]
+        runs.append(_engine(cfg, solid, src, samples, probes, rho, steps))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["f64", "b16"])
+@pytest.mark.parametrize("seed", SEEDS[:12])
+def test_random_scene_step_forms_agree(gpu, seed, precision, monkeypatch):
+    """The same steps through the replayed step graph, through direct launches
+    (NLOS_STEP_GRAPH=0) and through the allocation-free entry point, in uneven chunks with a
+    field capture: probes, field and populations bit for bit."""
+    cfg, solid, src, samples, probes, rho, steps = scenes.random_scene(seed, precision)
+    cfg.stream_pattern = nlos.PATTERN_TWO_ARRAY
+    chunks = [steps // 3, 1, steps - steps // 3 - 1]
+    runs = []
+    for form in ("graph", "direct", "noalloc"):
+        monkeypatch.setenv("NLOS_STEP_GRAPH", "0" if form == "direct" else "1")
+        e = nlos.Engine(cfg)
+        e.set_mask(solid)
+        if rho is not None:
+            e.set_density(rho)
+        e.set_source(src, samples)
+        e.set_probes(probes)
+        e.step(1, capture_field=True)      # a first capture allocates the field buffer
+        for n in chunks:
+            if form == "noalloc":
+                e.step_noalloc(n, capture_field=True)
+            else:
+                e.step(n, capture_field=True)
+        e.synchronize()
+        runs.append((e.read_probes(), e.read_field(), e.debug_read_f()[solid == 0]))
+        e.close()
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(a, b, equal_nan=True)
