@@ -253,8 +253,8 @@ def bench_city2048(args) -> None:
 
 def bench_batch(args) -> None:
     """256 independent FP32 scenes of uneven size, 3000 steps, all advanced together (one
-    multi-scene launch per kernel per step, two-step HIP graph replays, autotuned workgroup
-    shape).  Aggregate MLUPS over all scenes' cells; the same scenes stepped one engine at a
+    multi-scene launch per kernel per step, the sweep a persistent grid, two-step HIP graph
+    replays, autotuned tile shapes).  Aggregate MLUPS over all scenes' cells; the same scenes stepped one engine at a
     time (a bounded sample of steps) for comparison; a few scenes re-run alone and compared
     bit for bit."""
     steps = args.steps if args.steps != 200 else 3000
@@ -325,6 +325,8 @@ def bench_batch(args) -> None:
            "autotune": {"block": [tune.block_x, tune.block_y], "per_scene": bool(tune.per_scene),
                         "scenes_per_tile_width": width_count,
                         "candidates": tune.candidates, "sweep_ms": round(tune.best_ms, 4)},
+           "sweep_form": "persistent grid walking the (scene, tile) list"
+                         if os.environ.get("NLOS_BATCH_PERSISTENT", "1") != "0" else "one workgroup per tile",
            "graph_ms_per_step": round(tm.avg_ms, 4),
            "one_scene_at_a_time_mlups": round(seq_mlups, 1),
            "one_scene_at_a_time_sweep_ms_sum": round(seq_sweep_ms, 4),

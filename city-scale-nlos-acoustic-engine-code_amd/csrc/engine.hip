@@ -2320,6 +2320,10 @@ struct nlos_batch {
     std::vector<int> tile_w;              // per-scene sweep tile widths (64, 128, 256), or empty
     DevBuf step_args[2], ghost_args[2], solid_args[2], source_args[2];
     DevBuf tile_table, halo_blocks, solid_blocks, ctr;
+    DevBuf walk_items;                    // (scene, tile) work list of the persistent sweep
+    long long total_items = 0;
+    bool persistent = true;               // NLOS_BATCH_PERSISTENT=0: one workgroup per tile
+    int walk_per_cu = 32;                 // persistent grid: workgroups per compute unit
     long long total_tiles = 0;            // sweep grid x: the largest scene's tile count
     long long total_halo = 0, total_solid = 0, max_entries = 0;   // table sizes
     DevBuf sched_off, sched_scene, sched_cells, sched_adds;
@@ -2395,6 +2399,7 @@ int batch_prepare(nlos_batch *b, bool &moved) {
     std::vector<SourceArgs> sr[2];
     std::vector<int> tiles;                // sweep tiles per scene
     std::vector<int2> halo, solid;         // work tables (scene, first item)
+    std::vector<int2> walk;                // (scene, tile) of every sweep tile
     b->max_entries = 0;
     for (size_t si = 0; si < b->scenes.size(); ++si) {
         nlos_engine *e = b->scenes[si];
@@ -2413,6 +2418,8 @@ int batch_prepare(nlos_batch *b, bool &moved) {
         }
         const int scene = (int)tiles.size();
         tiles.push_back(st[0].back().nblocks);
+        if (b->persistent)
+            for (int t = 0; t < tiles.back(); ++t) walk.push_back(make_int2(scene, t));
         for (long long h = 0; h < halo_cells(e); h += 256) halo.push_back(make_int2(scene, (int)h));
         for (int q = 0; q < e->n_solid; q += 256) solid.push_back(make_int2(scene, q));
         b->max_entries += stencil_bound(e);
@@ -2424,6 +2431,8 @@ int batch_prepare(nlos_batch *b, bool &moved) {
         if (int rc = upload_vec(b, b->source_args[ph], sr[ph], moved)) return rc;
     }
     if (int rc = upload_vec(b, b->tile_table, tiles, moved)) return rc;
+    if (int rc = upload_vec(b, b->walk_items, walk, moved)) return rc;
+    b->total_items = (long long)walk.size();
     if (int rc = upload_vec(b, b->halo_blocks, halo, moved)) return rc;
     if (int rc = upload_vec(b, b->solid_blocks, solid, moved)) return rc;
     b->total_tiles = tiles.empty() ? 0 : *std::max_element(tiles.begin(), tiles.end());   // grid x
@@ -2459,7 +2468,10 @@ int batch_launch_step(nlos_batch *b, int phase) {
         if (r == hipSuccess)
             r = LL::solid_multi(b->solid_args[phase].as<SolidArgs>(), b->solid_blocks.as<int2>(),
                                 (int)b->total_solid, b->stream);
-        if (r == hipSuccess)
+        if (r == hipSuccess && b->persistent)
+            r = LL::step_ab_walk(b->step_args[phase].as<StepArgs>(), b->walk_items.as<int2>(), (int)b->total_items,
+                                 b->walk_per_cu * b->n_cu, b->stream);
+        else if (r == hipSuccess)
             r = LL::step_ab_multi(b->step_args[phase].as<StepArgs>(), b->tile_table.as<int>(), n,
                                   (int)b->total_tiles, b->stream);
         return r;
@@ -2547,6 +2559,8 @@ NLOS_API int nlos_batch_create(const nlos_config *cfgs, int32_t n, nlos_batch **
     b->mode = cfgs[0].precision;
     if (const char *x = std::getenv("NLOS_BATCH_BLOCK")) parse_block(x, b->block);
     if (const char *x = std::getenv("NLOS_BATCH_GRAPH")) b->use_graph = std::atoi(x) != 0;
+    if (const char *x = std::getenv("NLOS_BATCH_PERSISTENT")) b->persistent = std::atoi(x) != 0;
+    if (const char *x = std::getenv("NLOS_BATCH_WALK_PER_CU")) b->walk_per_cu = std::max(1, std::atoi(x));
     if (hipGetDevice(&b->device) != hipSuccess ||
         hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
         return bail(fail(NLOS_ERR_HIP, "device query failed"));
@@ -2584,8 +2598,8 @@ NLOS_API int nlos_batch_destroy(nlos_batch *b) {
     }
     DevBuf *bufs[] = {&b->step_args[0], &b->step_args[1], &b->ghost_args[0], &b->ghost_args[1],
                       &b->solid_args[0], &b->solid_args[1], &b->source_args[0], &b->source_args[1],
-                      &b->tile_table, &b->halo_blocks, &b->solid_blocks, &b->ctr, &b->sched_off,
-                      &b->sched_scene, &b->sched_cells, &b->sched_adds};
+                      &b->tile_table, &b->walk_items, &b->halo_blocks, &b->solid_blocks, &b->ctr,
+                      &b->sched_off, &b->sched_scene, &b->sched_cells, &b->sched_adds};
     for (DevBuf *d : bufs) d->release();
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -2625,20 +2639,27 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
     auto time_sweep = [&](const std::vector<int> &sel, const std::vector<int> &w, double &ms_out) -> int {
         std::vector<StepArgs> st;
         std::vector<int> tiles;
+        std::vector<int2> walk;
         for (size_t k = 0; k < sel.size(); ++k) {
             StepArgs a = multi_step_args(b, (size_t)sel[k], 0, w[k]);
             a.n_probe_cells = 0;
             a.fb[0] = a.fb[1] = nullptr;
             st.push_back(a);
             tiles.push_back(a.nblocks);
+            if (b->persistent)
+                for (int t = 0; t < a.nblocks; ++t) walk.push_back(make_int2((int)k, t));
         }
         bool mv = false;
         if (int rc = upload_vec(b, b->step_args[0], st, mv)) return rc;
         if (int rc = upload_vec(b, b->tile_table, tiles, mv)) return rc;
+        if (int rc = upload_vec(b, b->walk_items, walk, mv)) return rc;
         HIP_TRY(hipStreamSynchronize(b->stream));
         const int max_tiles = tiles.empty() ? 0 : *std::max_element(tiles.begin(), tiles.end());
-        auto sweep = [&]() {
+        auto sweep = [&]() {   // in the form the steps use
             return by_mode(b->mode, [&](auto L) {
+                if (b->persistent)
+                    return decltype(L)::step_ab_walk(b->step_args[0].as<StepArgs>(), b->walk_items.as<int2>(),
+                                                     (int)walk.size(), b->walk_per_cu * b->n_cu, b->stream);
                 return decltype(L)::step_ab_multi(b->step_args[0].as<StepArgs>(), b->tile_table.as<int>(),
                                                   (int)sel.size(), max_tiles, b->stream);
             });
@@ -2753,7 +2774,8 @@ NLOS_API int nlos_batch_step(nlos_batch *b, int64_t n) {
         HIP_TRY(hipMemcpyAsync(b->ctr.p, b->h_ctr, sizeof b->h_ctr, hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         // the tile shapes are read from the argument arrays; only the grids are baked
-        const std::vector<long long> shape = {b->total_tiles, b->total_halo, b->total_solid, b->max_entries};
+        const std::vector<long long> shape = {b->total_tiles, b->total_items, b->total_halo, b->total_solid,
+                                              b->max_entries};
         if (moved || shape != b->baked) drop_graph(b);
         if (m >= 2 && !b->exec && b->use_graph) {
             if (int rc = batch_capture(b)) return rc;

@@ -839,6 +839,21 @@ __global__ __launch_bounds__(256) void k_step_ab_multi(const StepArgs *__restric
                               a.tile_w, 256 >> lw);
 }
 
+// The persistent form of the multi-scene sweep: a grid sized to the device, whose
+// workgroups walk the (scene, tile) work list, workgroup b taking items b, b + gridDim.x, ...
+// (no workgroup of an empty tile, and one launch shape for any set of scenes).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_step_ab_walk(const StepArgs *__restrict__ scenes,
+                                                      const int2 *__restrict__ items, int n_items) {
+    for (int it = (int)blockIdx.x; it < n_items; it += (int)gridDim.x) {
+        const int2 w = items[it];
+        const StepArgs a = scenes[w.x];
+        const int lw = 31 - __builtin_clz((unsigned)a.tile_w);
+        step_ab_body<MODE, false>(a, w.y, (int)(threadIdx.x & ((unsigned)a.tile_w - 1u)), (int)(threadIdx.x >> lw),
+                                  a.tile_w, 256 >> lw);
+    }
+}
+
 // Wall copies of LAY_P (see Layout): for every fluid cell c next to a wall and every q
 // with c + e_q solid, the wall cell receives the reflected f*_q(c) (held in slot INV[q] of
 // c) in its slot q.  Needed after the state was set from outside.
@@ -1762,6 +1777,13 @@ hipError_t Launch<MODE>::fb_gather(const void *pop, const uint8_t *mask, const G
         else if (lay == LAY_Q) k_fb_gather<MODE, LAY_Q><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
         else k_fb_gather<MODE, LAY_AB><<<blocks_for(n, 256), 256, 0, s>>>(pop, mask, g, fb0, fb1, scale, inv_scale);
     }
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t Launch<MODE>::step_ab_walk(const StepArgs *scenes, const int2 *items, int n_items, int grid, hipStream_t s) {
+    if (n_items <= 0) return hipSuccess;
+    k_step_ab_walk<MODE><<<std::min(grid, n_items), 256, 0, s>>>(scenes, items, n_items);
     return hipGetLastError();
 }
 

@@ -43,6 +43,7 @@ struct Launch {
     // ghost and boundary-solid launches.
     static hipError_t step_ab_multi(const StepArgs *scenes, const int *n_tiles, int n_scenes, int max_tiles,
                                     hipStream_t s);
+    static hipError_t step_ab_walk(const StepArgs *scenes, const int2 *items, int n_items, int grid, hipStream_t s);
     static hipError_t ghost_multi(const GhostArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
     static hipError_t solid_multi(const SolidArgs *scenes, const int2 *blocks, int n_blocks, hipStream_t s);
     // Source entries of the step named by the device counter; at most max_entries per step.

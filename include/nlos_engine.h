@@ -386,7 +386,8 @@ int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result);
 int nlos_batch_tile_widths(const nlos_batch *b, int32_t *widths, int32_t n);
 int nlos_batch_set_tile_widths(nlos_batch *b, const int32_t *widths, int32_t n);
 /* n steps of every scene (each: apply_source(s*dt), exchange_halos, step_collision, probe
- * record, as nlos_step). */
+ * record, as nlos_step).  Each step is one launch per kernel over all scenes; the sweep's is
+ * a persistent grid of workgroups walking the (scene, tile) work list. */
 int nlos_batch_step(nlos_batch *b, int64_t n);
 int nlos_batch_synchronize(nlos_batch *b);
 /* When on, each nlos_batch_step call is bracketed by HIP events on the batch stream;

@@ -184,3 +184,25 @@ def test_direct_launches_equal_graph_replays(gpu, monkeypatch):
     for (pa, fa), (pb, fb) in zip(*runs):
         assert np.array_equal(pa, pb)
         assert np.array_equal(fa, fb, equal_nan=True)
+
+
+@pytest.mark.parametrize("precision", ["f32", "b16"])
+def test_persistent_sweep_equals_one_workgroup_per_tile(gpu, precision, monkeypatch):
+    """The persistent multi-scene sweep (a device-sized grid walking the (scene, tile) work
+    list) and NLOS_BATCH_PERSISTENT=0 (one workgroup per tile) give the same bits, with tile
+    widths that differ from scene to scene."""
+    defs = _defs(precision)
+    runs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("NLOS_BATCH_PERSISTENT", flag)
+        b = nlos.Batch([d[0] for d in defs])
+        for i, d in enumerate(defs):
+            _setup(b.scene(i), d)
+        b.set_tile_widths([128, 64, 256, 64, 128])
+        b.step(11)
+        b.synchronize()
+        runs.append([(b.scene(i).read_probes(), b.scene(i).debug_read_f()) for i in range(len(defs))])
+        b.close()
+    for (pa, fa), (pb, fb) in zip(*runs):
+        assert np.array_equal(pa, pb)
+        assert np.array_equal(fa, fb, equal_nan=True)
