@@ -2710,7 +2710,8 @@ NLOS_API int nlos_batch_autotune(nlos_batch *b, int32_t reps, nlos_tune *result)
             }
             for (int i : c) per[(size_t)i] = cw;
         }
-        if (per != std::vector<int>(n, best_w)) {
+        // (a choice of one width for every class is a single width, already timed above)
+        if (std::any_of(per.begin(), per.end(), [&](int w) { return w != per[0]; })) {
             if (int rc = time_sweep(all, per, per_ms)) return rc;
             use_per = per_ms < 0.99 * best;   // a gain above launch-to-launch noise
         }
