@@ -130,3 +130,37 @@ def test_random_scene_step_forms_agree(gpu, seed, precision, monkeypatch):
     for other in runs[1:]:
         for a, b in zip(runs[0], other):
             assert np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "f16s", "b16"])
+@pytest.mark.parametrize("seed", SEEDS[:8])
+def test_random_scene_checkpoint_restore(gpu, seed, precision):
+    """A checkpoint taken mid-run and restored into a fresh engine continues bit for bit."""
+    cfg, solid, src, samples, probes, rho, steps = scenes.random_scene(seed, precision)
+    if precision == "b16":
+        cfg.stream_pattern = nlos.PATTERN_TWO_ARRAY
+
+    def fresh():
+        e = nlos.Engine(cfg)
+        e.set_mask(solid)
+        if rho is not None:
+            e.set_density(rho)
+        e.set_source(src, samples)
+        e.set_probes(probes)
+        return e
+
+    a = fresh()
+    a.step(steps // 2)
+    blob = a.checkpoint()
+    a.step(steps - steps // 2, capture_field=True)
+    a.synchronize()
+    want = (a.read_probes(), a.read_field(), a.debug_read_f())
+    a.close()
+    r = fresh()
+    r.restore(blob)
+    r.step(steps - steps // 2, capture_field=True)
+    r.synchronize()
+    got = (r.read_probes(), r.read_field(), r.debug_read_f())
+    r.close()
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w, equal_nan=True)
