@@ -61,7 +61,9 @@ def log(msg: str) -> None:
 PATTERN_NAMES = {1: "inplace A-A", 2: "two-array pull"}
 
 
-def run_precision(wl, stl, precision, warmup, steps, timing=True, pattern="auto"):
+def run_precision(wl, stl, precision, warmup, steps, timing=True, pattern="auto", progress_every=0):
+    """One city run.  With progress_every > 0 the steps go in chunks of that many, each
+    followed by a progress line (stepping in chunks gives the same bits as one call)."""
     cfg = wl.config(precision)
     cfg.stream_pattern = nlos.PATTERNS[pattern]
     e = nlos.Engine(cfg)
@@ -73,8 +75,17 @@ def run_precision(wl, stl, precision, warmup, steps, timing=True, pattern="auto"
     e.step(warmup)
     e.synchronize()
     t0 = time.perf_counter()
-    e.step(steps)
-    e.synchronize()
+    if progress_every > 0:
+        done = 0
+        while done < steps:
+            n = min(progress_every, steps - done)
+            e.step(n)
+            e.synchronize()
+            done += n
+            log(f"    {precision}: {done}/{steps} steps ({time.perf_counter() - t0:.1f} s)")
+    else:
+        e.step(steps)
+        e.synchronize()
     wall = time.perf_counter() - t0
     tm = None
     if timing:
@@ -685,7 +696,7 @@ def main() -> None:
         for p in ["f64"] + [q for q in args.error_precisions.split(",") if q in res and q != "f64"]:
             t = time.perf_counter()
             traces[p] = run_precision(wl, stl, p, 0, args.error_steps, timing=False,
-                                      pattern=args.pattern)["probes"]
+                                      pattern=args.pattern, progress_every=2000)["probes"]
             log(f"  {p}: {args.error_steps} steps in {time.perf_counter() - t:.1f} s")
         ref = traces["f64"]
         err = {"steps": args.error_steps, "signal_max": float(np.abs(ref).max()), "bound": ERROR_BOUND}
